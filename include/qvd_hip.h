@@ -1,0 +1,151 @@
+/*
+ * qvd_hip.h — C ABI of the MI355X state-vector engine (libqvd_hip.so).
+ *
+ * This is the seam a Rust `qvd` crate binds through `extern "C"` in place of
+ * its CPU state-vector backend. Every function returns an int status (0 = ok,
+ * negative = error) and never aborts; the message of the last error on the
+ * calling thread is qvd_last_error(). Calls are synchronous: when a function
+ * returns, its GPU work is complete.
+ *
+ * Conventions (kept from the reference):
+ *   - qubit q is bit q of an amplitude index (src/state.rs:12-16);
+ *   - a matrix on targets [q0..q(k-1)] is row-major 2^k x 2^k complex128
+ *     ((re, im) pairs), index bit j = qubit qj (src/matrix.rs:1-18);
+ *   - controls are (qubit, value) pairs (src/fusion.rs:20-21);
+ *   - amplitudes cross the ABI as complex128 in logical index order
+ *     (src/state.rs:146-159; python/src/lib.rs:703-716).
+ *
+ * Ownership: the library owns device memory; the caller owns every host
+ * array and keeps it alive for the duration of the call. A qvd_sv handle is
+ * not safe for concurrent use (the reference's `&mut self`,
+ * src/state.rs:181); distinct handles may be used from distinct threads.
+ */
+#ifndef QVD_HIP_H
+#define QVD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define QVD_OK 0
+#define QVD_EINVAL (-1)        /* invalid argument (the reference panics here) */
+#define QVD_ENOMEM (-2)        /* device or host allocation failed */
+#define QVD_EHIP (-3)          /* a HIP runtime call failed */
+#define QVD_EUNSUPPORTED (-4)  /* gate wider than the kernels handle */
+
+/* Amplitude precision: the reference's `Real` parameter (src/simd.rs:15-23). */
+#define QVD_C64 0  /* complex64 ("single") */
+#define QVD_C128 1 /* complex128 ("double") */
+
+/* Planner choice for qvd_sv_apply_ops. */
+#define QVD_PLANNER_HEURISTIC 0 /* restated fusion.rs + blocking.rs stage planning */
+#define QVD_PLANNER_LEARNED 1   /* MLP cost model over candidate plans, heuristic fallback */
+
+typedef struct qvd_sv qvd_sv; /* opaque: HBM amplitudes + a HIP stream */
+
+/* One unitary op: `matrix` on `targets`, applied where every control holds.
+ * Mirrors fusion::Op (src/fusion.rs:16-56). control_values may be NULL
+ * (all controls true). */
+typedef struct qvd_op {
+    const uint32_t *targets;
+    uint32_t num_targets;
+    const uint32_t *controls;
+    const uint8_t *control_values;
+    uint32_t num_controls;
+    const double *matrix;
+} qvd_op;
+
+/* Options of qvd_sv_apply_ops: simulator::Options (src/simulator.rs:37-66)
+ * restricted to the state vector, plus the tile and planner knobs. */
+typedef struct qvd_options {
+    uint32_t max_fused_qubits; /* largest fused gate; 1 disables fusion (default 4) */
+    int32_t tile_bits;         /* qubits per LDS-resident tile; <0 = no blocking, 0 = default */
+    uint32_t planner;          /* QVD_PLANNER_* */
+    uint32_t reserved;
+} qvd_options;
+
+/* What a run did: simulator::Stats (src/simulator.rs:70-93). */
+typedef struct qvd_stats {
+    uint64_t gates;   /* ops in */
+    uint64_t kernels; /* ops after fusion */
+    uint64_t passes;  /* full HBM passes: one per stage and per full-state op */
+    uint64_t stages;  /* tile stages launched */
+    double gate_seconds;
+    double measure_seconds;
+    double plan_seconds; /* host planning inside gate_seconds */
+} qvd_stats;
+
+const char *qvd_last_error(void);
+
+/* Library / device information: number of visible devices and the name of
+ * the architecture of device 0 (e.g. "gfx950"). */
+int qvd_device_info(int *num_devices, char *arch, uint32_t arch_capacity);
+
+/* StateVector::zero (src/state.rs:39-49): |0...0> in HBM. */
+int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out);
+int qvd_sv_destroy(qvd_sv *sv);
+int qvd_sv_num_qubits(const qvd_sv *sv, uint32_t *out);
+int qvd_sv_precision(const qvd_sv *sv, int *out);
+/* Reset to |0...0> without reallocating. */
+int qvd_sv_set_zero(qvd_sv *sv);
+
+/* StateVector::from_amplitudes / set (src/state.rs:52-63, 137-143):
+ * `amplitudes` holds 2^n complex128 values in logical order. */
+int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len);
+/* StateVector::to_vec (src/state.rs:146-159), widened to complex128. */
+int qvd_sv_get_amplitudes(const qvd_sv *sv, double *amplitudes, uint64_t len);
+
+/* StateVector::apply (src/state.rs:181-198): one op in one HBM pass
+ * (controls restrict the pass to the controlled subspace). */
+int qvd_sv_apply(qvd_sv *sv, const double *matrix, const uint32_t *targets, uint32_t num_targets,
+                 const uint32_t *controls, const uint8_t *control_values, uint32_t num_controls);
+
+/* simulator::apply_ops (src/simulator.rs:122-147) with blocking::apply_blocked
+ * (src/blocking.rs:230-302): fuse, plan LDS-resident stages, launch. */
+int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options *options,
+                     qvd_stats *stats);
+
+/* StateVector::norm_sqr (src/state.rs:172-177), accumulated in FP64. */
+int qvd_sv_norm_sqr(const qvd_sv *sv, double *out);
+/* StateVector::scale (src/state.rs:201-207). */
+int qvd_sv_scale(qvd_sv *sv, double factor);
+/* probability_one (src/measure.rs:43-65), FP64. */
+int qvd_sv_probability_one(const qvd_sv *sv, uint32_t qubit, double *out);
+/* probabilities (src/measure.rs:25-40): 2^n FP64 values in logical order. */
+int qvd_sv_probabilities(const qvd_sv *sv, double *out, uint64_t len);
+/* <Z_q> for every qubit (n values) and <Z_q Z_{q+1}> (n-1 values), FP64.
+ * New in this engine; <Z_q> = 1 - 2 probability_one(q). */
+int qvd_sv_expect_z(const qvd_sv *sv, double *z, double *zz);
+/* measure (src/measure.rs:68-80) with the uniform draw supplied by the
+ * caller: outcome = uniform < P(1); the state collapses and renormalises. */
+int qvd_sv_measure(qvd_sv *sv, uint32_t qubit, double uniform, uint8_t *outcome);
+/* reset (src/measure.rs:83-87): measure, then X on outcome 1. */
+int qvd_sv_reset(qvd_sv *sv, uint32_t qubit, double uniform);
+/* sample_unordered (src/measure.rs:104-174) with the caller's uniforms in
+ * [0, 1): FP64 chunk sums in index order (chunks of 32768 amplitudes for
+ * complex64, 16384 for complex128), sequential prefix, targets u * total in
+ * ascending order, strict-< walk. out[i] is the logical basis index of the
+ * i-th smallest target; bit-exact with the reference's algorithm on the same
+ * amplitudes and uniforms. */
+int qvd_sv_sample(const qvd_sv *sv, const double *uniforms, uint64_t shots, uint64_t *out);
+
+/* Sampling statistics of the last qvd_sv_sample on this handle (seconds). */
+int qvd_sv_last_sample_seconds(const qvd_sv *sv, double *out);
+
+/* Gate::matrix (src/circuit.rs:133-202) by OpenQASM name: the full matrix
+ * on the gate's qubits (controls first, on the low index bits). `matrix`
+ * receives 4^k (re, im) pairs; capacity counts complex entries. */
+int qvd_gate_matrix(const char *name, const double *params, uint32_t num_params, double *matrix, uint32_t capacity,
+                    uint32_t *num_qubits);
+/* gate_op (src/simulator.rs:106-119): the number of leading control qubits
+ * and the matrix of the gate applied to the remaining qubits. */
+int qvd_gate_lower(const char *name, const double *params, uint32_t num_params, uint32_t *num_controls,
+                   uint32_t *num_targets, double *matrix, uint32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QVD_HIP_H */

@@ -1,0 +1,682 @@
+// The C ABI of the state-vector engine (include/qvd_hip.h): state handles in
+// HBM, single ops, planned op sequences, reductions, measurement and
+// sampling. Host-side validation turns every assert of the reference into a
+// status code.
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "common.h"
+#include "kernels.h"
+#include "plan.h"
+
+using namespace qvd;
+
+namespace qvd {
+
+thread_local std::string g_last_error;
+void set_last_error(const std::string &m) { g_last_error = m; }
+
+namespace {
+
+// An owned device allocation.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    explicit DBuf(size_t b) { alloc(b); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t b) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        if (b == 0) return;
+        QVD_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <typename Fn>
+int guard(Fn &&fn) {
+    try {
+        fn();
+        return QVD_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("out of host memory");
+        return QVD_ENOMEM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return QVD_EINVAL;
+    }
+}
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+}  // namespace qvd
+
+struct qvd_sv {
+    uint32_t n = 0;
+    int precision = QVD_C64;
+    int device = 0;
+    uint64_t len = 0;
+    DBuf amps;
+    hipStream_t stream = nullptr;
+    DBuf scratch;  // small reduction buffers
+    double last_sample_seconds = 0;
+
+    size_t elem() const { return precision == QVD_C64 ? sizeof(float2) : sizeof(double2); }
+    void bind() const { QVD_HIP(hipSetDevice(device)); }
+    void sync() const { QVD_HIP(hipStreamSynchronize(stream)); }
+    double *scratch_doubles(size_t count) {
+        if (scratch.bytes < count * sizeof(double)) scratch.alloc(std::max<size_t>(count * sizeof(double), 1 << 16));
+        return scratch.as<double>();
+    }
+};
+
+namespace qvd {
+namespace {
+
+template <typename F>
+auto with_type(const qvd_sv *sv, F &&f) {
+    if (sv->precision == QVD_C64) return f((float2 *)sv->amps.p);
+    return f((double2 *)sv->amps.p);
+}
+
+void check_handle(const qvd_sv *sv) { require(sv != nullptr, "null state handle"); }
+
+// Validates qubits of an op the way the reference asserts them
+// (src/state.rs:182-188; python/src/lib.rs:29-43).
+void check_op_qubits(uint32_t n, const std::vector<uint32_t> &targets,
+                     const std::vector<std::pair<uint32_t, bool>> &controls) {
+    std::vector<uint32_t> all = targets;
+    for (auto &c : controls) all.push_back(c.first);
+    for (uint32_t q : all) require(q < n, "qubit " + std::to_string(q) + " out of range for " + std::to_string(n) + " qubits");
+    std::vector<uint32_t> s = all;
+    std::sort(s.begin(), s.end());
+    require(std::adjacent_find(s.begin(), s.end()) == s.end(), "an op cannot act on the same qubit twice");
+}
+
+template <typename T2> T2 to_t2(C64 v) {
+    T2 r;
+    r.x = (decltype(r.x))v.real();
+    r.y = (decltype(r.y))v.imag();
+    return r;
+}
+
+// One op as its own HBM pass: the dense kernel over the controlled subspace,
+// or the diagonal kernel.
+template <typename T2>
+void apply_full(qvd_sv *sv, T2 *s, const Op &op) {
+    const uint32_t k = (uint32_t)op.targets.size();
+    if (k == 0) return;
+    if (k > MAX_GATE_QUBITS)
+        fail(QVD_EUNSUPPORTED, "ops on more than " + std::to_string(MAX_GATE_QUBITS) + " target qubits are not supported");
+    uint64_t cmask = 0, cval = 0;
+    for (auto &c : op.controls) {
+        cmask |= uint64_t(1) << c.first;
+        if (c.second) cval |= uint64_t(1) << c.first;
+    }
+    if (op.diagonal) {
+        std::vector<T2> e(size_t(1) << k);
+        for (size_t i = 0; i < e.size(); i++) e[i] = to_t2<T2>(op.m.at(i, i));
+        DBuf dev(e.size() * sizeof(T2));
+        QVD_HIP(hipMemcpyAsync(dev.p, e.data(), dev.bytes, hipMemcpyHostToDevice, sv->stream));
+        DiagTargets t{};
+        for (uint32_t j = 0; j < k; j++) t.q[j] = op.targets[j];
+        launch_diag<T2>(s, sv->len, k, t, cmask, cval, dev.as<T2>(), sv->stream);
+        QVD_HIP(hipGetLastError());
+        sv->sync();
+        return;
+    }
+    GateDesc d{};
+    d.k = k;
+    std::vector<uint32_t> ins = op.targets;
+    for (auto &c : op.controls) ins.push_back(c.first);
+    std::sort(ins.begin(), ins.end());
+    d.nins = (uint32_t)ins.size();
+    for (size_t j = 0; j < ins.size(); j++) d.ins[j] = ins[j];
+    for (uint32_t j = 0; j < (1u << k); j++) {
+        uint64_t o = 0;
+        for (uint32_t b = 0; b < k; b++) o |= uint64_t((j >> b) & 1) << op.targets[b];
+        d.offsets[j] = o;
+    }
+    d.ctrl_val = cval;
+    d.iterations = uint64_t(1) << (sv->n - d.nins);
+    std::vector<T2> m(op.m.a.size());
+    for (size_t i = 0; i < m.size(); i++) m[i] = to_t2<T2>(op.m.a[i]);
+    DBuf dev(m.size() * sizeof(T2));
+    QVD_HIP(hipMemcpyAsync(dev.p, m.data(), dev.bytes, hipMemcpyHostToDevice, sv->stream));
+    launch_dense<T2>(s, d, dev.as<T2>(), sv->stream);
+    QVD_HIP(hipGetLastError());
+    sv->sync();
+}
+
+Op make_op(uint32_t n, const double *matrix, const uint32_t *targets, uint32_t k, const uint32_t *controls,
+           const uint8_t *values, uint32_t c) {
+    require(k == 0 || (targets && matrix), "null targets or matrix");
+    require(c == 0 || controls, "null controls");
+    Op op;
+    op.targets.assign(targets, targets + k);
+    for (uint32_t j = 0; j < c; j++) op.controls.push_back({controls[j], values ? values[j] != 0 : true});
+    check_op_qubits(n, op.targets, op.controls);
+    require(k <= 20, "op too wide");
+    if (k > 0) op.m = HMatrix::from_interleaved(matrix, k);
+    else op.m = HMatrix::identity(0);
+    op.diagonal = op.m.is_diagonal();
+    return op;
+}
+
+// Tile sizes: 2^13 amplitudes (64 KiB of LDS) in complex64, 2^12 in
+// complex128; the lowest 5 / 4 qubits always in the tile (256-byte runs).
+struct TileConfig {
+    uint32_t L, contig;
+};
+TileConfig default_tiles(int precision) { return precision == QVD_C64 ? TileConfig{13, 5} : TileConfig{12, 4}; }
+
+// Lowers a planned stage into device descriptors.
+template <typename T2>
+void lower_stage(const Stage &st, uint32_t n, std::vector<StageOp> &ops, std::vector<T2> &coefs, StageDesc &d) {
+    std::vector<int> tile_bit(n, -1);
+    for (size_t j = 0; j < st.local.size(); j++) tile_bit[st.local[j]] = (int)j;
+    d = StageDesc{};
+    d.L = (uint32_t)st.local.size();
+    for (uint32_t j = 0; j < d.L; j++) d.local_pos[j] = st.local[j];
+    d.nglob = 0;
+    for (uint32_t q = 0; q < n; q++)
+        if (tile_bit[q] < 0) d.glob_pos[d.nglob++] = q;
+    d.num_tiles = uint64_t(1) << d.nglob;
+    d.op_off = (uint32_t)ops.size();
+    d.num_ops = 0;
+    for (const Op &op : st.ops) {
+        if (op.targets.empty()) continue;
+        StageOp so{};
+        so.k = (uint32_t)op.targets.size();
+        std::vector<uint32_t> ins;
+        for (auto &c : op.controls) {
+            if (tile_bit[c.first] >= 0) {
+                so.lc_mask |= 1u << tile_bit[c.first];
+                if (c.second) so.lc_val |= 1u << tile_bit[c.first];
+                ins.push_back((uint32_t)tile_bit[c.first]);
+            } else {
+                so.gc_mask |= uint64_t(1) << c.first;
+                if (c.second) so.gc_val |= uint64_t(1) << c.first;
+            }
+        }
+        so.coef_off = (uint32_t)coefs.size();
+        if (op.diagonal) {
+            so.kind = OP_DIAG;
+            for (uint32_t j = 0; j < so.k; j++) {
+                const uint32_t q = op.targets[j];
+                so.tgt[j] = tile_bit[q] >= 0 ? (uint32_t)tile_bit[q] : (GLOBAL_BIT | q);
+            }
+            for (size_t i = 0; i < op.m.dim(); i++) coefs.push_back(to_t2<T2>(op.m.at(i, i)));
+        } else {
+            so.kind = OP_DENSE;
+            for (uint32_t j = 0; j < so.k; j++) {
+                const int b = tile_bit[op.targets[j]];
+                require(b >= 0, "internal: dense stage op on a non-tile qubit");
+                so.tgt[j] = (uint32_t)b;
+                ins.push_back((uint32_t)b);
+            }
+            for (const C64 &v : op.m.a) coefs.push_back(to_t2<T2>(v));
+        }
+        std::sort(ins.begin(), ins.end());
+        so.nins = (uint32_t)ins.size();
+        for (size_t j = 0; j < ins.size(); j++) so.ins[j] = ins[j];
+        ops.push_back(so);
+        d.num_ops++;
+    }
+}
+
+int g_num_cus = 0;
+int num_cus(int device) {
+    if (g_num_cus == 0) {
+        hipDeviceProp_t p;
+        QVD_HIP(hipGetDeviceProperties(&p, device));
+        g_num_cus = p.multiProcessorCount;
+    }
+    return g_num_cus;
+}
+
+template <typename T2>
+void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st) {
+    // Lower every stage first so one upload serves the whole plan.
+    std::vector<StageOp> sops;
+    std::vector<T2> coefs;
+    std::vector<StageDesc> descs;
+    for (const Step &step : steps) {
+        if (!step.is_stage) continue;
+        StageDesc d;
+        lower_stage<T2>(step.stage, sv->n, sops, coefs, d);
+        descs.push_back(d);
+    }
+    DBuf dops(std::max<size_t>(sops.size(), 1) * sizeof(StageOp));
+    DBuf dcoef(std::max<size_t>(coefs.size(), 1) * sizeof(T2));
+    if (!sops.empty()) {
+        QVD_HIP(hipMemcpyAsync(dops.p, sops.data(), sops.size() * sizeof(StageOp), hipMemcpyHostToDevice, sv->stream));
+        QVD_HIP(hipMemcpyAsync(dcoef.p, coefs.data(), coefs.size() * sizeof(T2), hipMemcpyHostToDevice, sv->stream));
+    }
+    const int cus = num_cus(sv->device);
+    size_t si = 0;
+    for (const Step &step : steps) {
+        if (step.is_stage) {
+            const StageDesc &d = descs[si++];
+            st.kernels += d.num_ops;
+            if (d.num_ops == 0) continue;
+            const size_t lds = stage_lds_bytes<T2>(d.L);
+            const unsigned per_cu = lds <= 80 * 1024 ? 2 : 1;
+            const uint64_t want = (uint64_t)cus * per_cu * 2;
+            const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, want);
+            launch_stage<T2>(s, d, dops.as<StageOp>(), dcoef.as<T2>(), grid, sv->stream);
+            QVD_HIP(hipGetLastError());
+            st.stages++;
+            st.passes++;
+        } else {
+            apply_full<T2>(sv, s, step.op);
+            st.kernels++;
+            st.passes++;
+        }
+    }
+    sv->sync();
+}
+
+uint64_t state_cap_bytes() {
+    const char *e = std::getenv("QVD_MAX_STATE_BYTES");
+    if (!e || !*e) return 0;
+    return std::strtoull(e, nullptr, 10);
+}
+
+}  // namespace
+}  // namespace qvd
+
+extern "C" {
+
+const char *qvd_last_error(void) { return qvd::g_last_error.c_str(); }
+
+int qvd_device_info(int *num_devices, char *arch, uint32_t arch_capacity) {
+    return guard([&] {
+        require(num_devices != nullptr, "null output");
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess) count = 0;
+        *num_devices = count;
+        if (arch && arch_capacity) {
+            arch[0] = 0;
+            if (count > 0) {
+                hipDeviceProp_t p;
+                QVD_HIP(hipGetDeviceProperties(&p, 0));
+                std::strncpy(arch, p.gcnArchName, arch_capacity - 1);
+                arch[arch_capacity - 1] = 0;
+            }
+        }
+    });
+}
+
+int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out) {
+    return guard([&] {
+        require(out != nullptr, "null output handle");
+        *out = nullptr;
+        require(precision == QVD_C64 || precision == QVD_C128, "precision must be QVD_C64 or QVD_C128");
+        require(num_qubits <= 34, "a state vector of more than 34 qubits does not fit in memory");
+        std::unique_ptr<qvd_sv> sv(new qvd_sv());
+        sv->n = num_qubits;
+        sv->precision = precision;
+        sv->len = uint64_t(1) << num_qubits;
+        const uint64_t bytes = sv->len * sv->elem();
+        const uint64_t cap = state_cap_bytes();
+        if (cap && bytes > cap)
+            fail(QVD_ENOMEM, "state of " + std::to_string(bytes) + " bytes exceeds QVD_MAX_STATE_BYTES");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(QVD_EHIP, "no HIP device is visible");
+        QVD_HIP(hipGetDevice(&sv->device));
+        sv->bind();
+        QVD_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+        sv->amps.alloc(bytes);
+        with_type(sv.get(), [&](auto *s) {
+            launch_init_zero(s, sv->len, sv->stream);
+            return 0;
+        });
+        QVD_HIP(hipGetLastError());
+        sv->sync();
+        *out = sv.release();
+    });
+}
+
+int qvd_sv_destroy(qvd_sv *sv) {
+    return guard([&] {
+        if (!sv) return;
+        sv->bind();
+        if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+        sv->amps.alloc(0);
+        sv->scratch.alloc(0);
+        if (sv->stream) (void)hipStreamDestroy(sv->stream);
+        delete sv;
+    });
+}
+
+int qvd_sv_num_qubits(const qvd_sv *sv, uint32_t *out) {
+    return guard([&] {
+        check_handle(sv);
+        *out = sv->n;
+    });
+}
+
+int qvd_sv_precision(const qvd_sv *sv, int *out) {
+    return guard([&] {
+        check_handle(sv);
+        *out = sv->precision;
+    });
+}
+
+int qvd_sv_set_zero(qvd_sv *sv) {
+    return guard([&] {
+        check_handle(sv);
+        sv->bind();
+        with_type(sv, [&](auto *s) {
+            launch_init_zero(s, sv->len, sv->stream);
+            return 0;
+        });
+        QVD_HIP(hipGetLastError());
+        sv->sync();
+    });
+}
+
+int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len) {
+    return guard([&] {
+        check_handle(sv);
+        require(amplitudes != nullptr, "null amplitudes");
+        require(len == sv->len, "expected " + std::to_string(sv->len) + " amplitudes, got " + std::to_string(len));
+        sv->bind();
+        const uint64_t chunk = std::min<uint64_t>(len, uint64_t(1) << 24);
+        DBuf stage(chunk * sizeof(double2));
+        for (uint64_t off = 0; off < len; off += chunk) {
+            const uint64_t cnt = std::min(chunk, len - off);
+            QVD_HIP(hipMemcpyAsync(stage.p, amplitudes + 2 * off, cnt * sizeof(double2), hipMemcpyHostToDevice, sv->stream));
+            with_type(sv, [&](auto *s) {
+                launch_narrow(stage.as<double2>(), s + off, cnt, sv->stream);
+                return 0;
+            });
+            QVD_HIP(hipGetLastError());
+            sv->sync();
+        }
+    });
+}
+
+int qvd_sv_get_amplitudes(const qvd_sv *sv, double *amplitudes, uint64_t len) {
+    return guard([&] {
+        check_handle(sv);
+        require(amplitudes != nullptr, "null amplitudes");
+        require(len == sv->len, "expected " + std::to_string(sv->len) + " amplitudes, got " + std::to_string(len));
+        sv->bind();
+        const uint64_t chunk = std::min<uint64_t>(len, uint64_t(1) << 24);
+        DBuf stage(chunk * sizeof(double2));
+        for (uint64_t off = 0; off < len; off += chunk) {
+            const uint64_t cnt = std::min(chunk, len - off);
+            with_type(sv, [&](auto *s) {
+                launch_widen(s + off, stage.as<double2>(), cnt, sv->stream);
+                return 0;
+            });
+            QVD_HIP(hipGetLastError());
+            QVD_HIP(hipMemcpyAsync(amplitudes + 2 * off, stage.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, sv->stream));
+            sv->sync();
+        }
+    });
+}
+
+int qvd_sv_apply(qvd_sv *sv, const double *matrix, const uint32_t *targets, uint32_t num_targets,
+                 const uint32_t *controls, const uint8_t *control_values, uint32_t num_controls) {
+    return guard([&] {
+        check_handle(sv);
+        Op op = make_op(sv->n, matrix, targets, num_targets, controls, control_values, num_controls);
+        sv->bind();
+        with_type(sv, [&](auto *s) {
+            apply_full(sv, s, op);
+            return 0;
+        });
+    });
+}
+
+int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options *options, qvd_stats *stats) {
+    return guard([&] {
+        check_handle(sv);
+        require(num_ops == 0 || ops != nullptr, "null ops");
+        const double t0 = now();
+        qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
+        if (options) opt = *options;
+        std::vector<Op> v;
+        v.reserve(num_ops);
+        for (uint64_t i = 0; i < num_ops; i++) {
+            const qvd_op &o = ops[i];
+            Op op = make_op(sv->n, o.matrix, o.targets, o.num_targets, o.controls, o.control_values, o.num_controls);
+            if (op.targets.size() > MAX_GATE_QUBITS)
+                fail(QVD_EUNSUPPORTED, "ops on more than " + std::to_string(MAX_GATE_QUBITS) + " target qubits are not supported");
+            v.push_back(std::move(op));
+        }
+        std::vector<Step> steps;
+        if (opt.tile_bits < 0) {
+            steps = plan_unblocked(v, opt.max_fused_qubits);
+        } else {
+            TileConfig tc = default_tiles(sv->precision);
+            PlanParams p;
+            p.num_qubits = sv->n;
+            p.tile_bits = opt.tile_bits > 0 ? (uint32_t)opt.tile_bits : tc.L;
+            require(p.tile_bits <= 14 - (sv->precision == QVD_C128 ? 1 : 0) && p.tile_bits >= 1,
+                    "tile_bits out of range (1..14 for complex64, 1..13 for complex128)");
+            p.contig = std::min(tc.contig, p.tile_bits);
+            p.max_fused = std::max<uint32_t>(opt.max_fused_qubits, 1);
+            p.max_stage_op = 4;
+            steps = plan_heuristic(v, p);
+        }
+        const double t1 = now();
+        qvd_stats st{};
+        st.gates = num_ops;
+        sv->bind();
+        with_type(sv, [&](auto *s) {
+            run_plan(sv, s, steps, st);
+            return 0;
+        });
+        const double t2 = now();
+        st.plan_seconds = t1 - t0;
+        st.gate_seconds = t2 - t0;
+        if (stats) *stats = st;
+    });
+}
+
+int qvd_sv_norm_sqr(const qvd_sv *sv_c, double *out) {
+    return guard([&] {
+        check_handle(sv_c);
+        require(out != nullptr, "null output");
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        sv->bind();
+        const unsigned blocks = 1024;
+        double *buf = sv->scratch_doubles(blocks + 1);
+        with_type(sv, [&](auto *s) {
+            launch_masked_norm(s, sv->len, 0, 0, buf, blocks, buf + blocks, sv->stream);
+            return 0;
+        });
+        QVD_HIP(hipGetLastError());
+        QVD_HIP(hipMemcpyAsync(out, buf + blocks, sizeof(double), hipMemcpyDeviceToHost, sv->stream));
+        sv->sync();
+    });
+}
+
+int qvd_sv_scale(qvd_sv *sv, double factor) {
+    return guard([&] {
+        check_handle(sv);
+        sv->bind();
+        with_type(sv, [&](auto *s) {
+            launch_scale(s, sv->len, factor, sv->stream);
+            return 0;
+        });
+        QVD_HIP(hipGetLastError());
+        sv->sync();
+    });
+}
+
+static double probability_one_impl(qvd_sv *sv, uint32_t qubit) {
+    require(qubit < sv->n, "qubit " + std::to_string(qubit) + " out of range for " + std::to_string(sv->n) + " qubits");
+    const unsigned blocks = 1024;
+    double *buf = sv->scratch_doubles(blocks + 1);
+    const uint64_t m = uint64_t(1) << qubit;
+    with_type(sv, [&](auto *s) {
+        launch_masked_norm(s, sv->len, m, m, buf, blocks, buf + blocks, sv->stream);
+        return 0;
+    });
+    QVD_HIP(hipGetLastError());
+    double r = 0;
+    QVD_HIP(hipMemcpyAsync(&r, buf + blocks, sizeof(double), hipMemcpyDeviceToHost, sv->stream));
+    sv->sync();
+    return r;
+}
+
+int qvd_sv_probability_one(const qvd_sv *sv_c, uint32_t qubit, double *out) {
+    return guard([&] {
+        check_handle(sv_c);
+        require(out != nullptr, "null output");
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        sv->bind();
+        *out = probability_one_impl(sv, qubit);
+    });
+}
+
+int qvd_sv_probabilities(const qvd_sv *sv, double *out, uint64_t len) {
+    return guard([&] {
+        check_handle(sv);
+        require(out != nullptr, "null output");
+        require(len == sv->len, "length does not match the state");
+        std::vector<double> a(2 * len);
+        int rc = qvd_sv_get_amplitudes(sv, a.data(), len);
+        if (rc != QVD_OK) fail(rc, qvd::g_last_error);
+        for (uint64_t i = 0; i < len; i++) out[i] = a[2 * i] * a[2 * i] + a[2 * i + 1] * a[2 * i + 1];
+    });
+}
+
+int qvd_sv_expect_z(const qvd_sv *sv_c, double *z, double *zz) {
+    return guard([&] {
+        check_handle(sv_c);
+        require(z != nullptr, "null output");
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        const uint32_t n = sv->n;
+        if (n == 0) return;
+        sv->bind();
+        const uint32_t B = expect_segment_bits(n);
+        const uint64_t segs = uint64_t(1) << (n - B);
+        DBuf seg(segs * 2 * B * sizeof(double));
+        DBuf res((2 * n) * sizeof(double));
+        with_type(sv, [&](auto *s) {
+            launch_expect_z(s, n, seg.as<double>(), res.as<double>(), res.as<double>() + n, sv->stream);
+            return 0;
+        });
+        QVD_HIP(hipGetLastError());
+        std::vector<double> h(2 * n);
+        QVD_HIP(hipMemcpyAsync(h.data(), res.p, (2 * n - 1) * sizeof(double), hipMemcpyDeviceToHost, sv->stream));
+        sv->sync();
+        std::copy(h.begin(), h.begin() + n, z);
+        if (zz) std::copy(h.begin() + n, h.begin() + 2 * n - 1, zz);
+    });
+}
+
+static uint8_t measure_impl(qvd_sv *sv, uint32_t qubit, double uniform) {
+    double p1 = probability_one_impl(sv, qubit);
+    p1 = std::min(std::max(p1, 0.0), 1.0);
+    const bool outcome = uniform < p1;
+    const double p = outcome ? p1 : 1.0 - p1;
+    const double keep = 1.0 / std::sqrt(p);
+    with_type(sv, [&](auto *s) {
+        launch_collapse(s, sv->len, qubit, outcome ? 1u : 0u, keep, sv->stream);
+        return 0;
+    });
+    QVD_HIP(hipGetLastError());
+    sv->sync();
+    return outcome ? 1 : 0;
+}
+
+int qvd_sv_measure(qvd_sv *sv, uint32_t qubit, double uniform, uint8_t *outcome) {
+    return guard([&] {
+        check_handle(sv);
+        require(outcome != nullptr, "null output");
+        sv->bind();
+        *outcome = measure_impl(sv, qubit, uniform);
+    });
+}
+
+int qvd_sv_reset(qvd_sv *sv, uint32_t qubit, double uniform) {
+    return guard([&] {
+        check_handle(sv);
+        sv->bind();
+        if (measure_impl(sv, qubit, uniform)) {
+            Op x;
+            x.targets = {qubit};
+            x.m = HMatrix(1);
+            x.m.at(0, 1) = C64(1, 0);
+            x.m.at(1, 0) = C64(1, 0);
+            x.diagonal = false;
+            with_type(sv, [&](auto *s) {
+                apply_full(sv, s, x);
+                return 0;
+            });
+        }
+    });
+}
+
+int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, uint64_t *out) {
+    return guard([&] {
+        check_handle(sv_c);
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        if (shots == 0) return;
+        require(uniforms != nullptr && out != nullptr, "null uniforms or output");
+        require(shots < (uint64_t(1) << 31), "at most 2^31 - 1 shots per call");
+        sv->bind();
+        const double t0 = now();
+        // Chunks of the reference's CHUNK_BLOCKS blocks (src/measure.rs:12-13):
+        // 4096 x 8 amplitudes in complex64, 4096 x 4 in complex128.
+        const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
+        const uint64_t nc = (sv->len + chunk - 1) / chunk;
+        SampleScratch w;
+        DBuf sums(nc * sizeof(double)), prefix((nc + 1) * sizeof(double)), bounds((nc + 1) * sizeof(uint64_t));
+        DBuf tu(shots * sizeof(double)), ts(shots * sizeof(double)), du(shots * sizeof(double));
+        DBuf dout(shots * sizeof(uint64_t));
+        w.sort_temp_bytes = sort_temp_bytes(shots);
+        DBuf temp(std::max<size_t>(w.sort_temp_bytes, 16));
+        w.sums = sums.as<double>();
+        w.prefix = prefix.as<double>();
+        w.bounds = bounds.as<uint64_t>();
+        w.targets_unsorted = tu.as<double>();
+        w.targets = ts.as<double>();
+        w.sort_temp = temp.p;
+        QVD_HIP(hipMemcpyAsync(du.p, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
+        require(w.sort_temp_bytes > 0, "radix sort temporary size query failed");
+        with_type(sv, [&](auto *s) {
+            QVD_HIP(launch_sample(s, sv->len, chunk, du.as<double>(), shots, w, dout.as<uint64_t>(), sv->stream));
+            return 0;
+        });
+        QVD_HIP(hipMemcpyAsync(out, dout.p, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
+        sv->sync();
+        sv->last_sample_seconds = now() - t0;
+    });
+}
+
+int qvd_sv_last_sample_seconds(const qvd_sv *sv, double *out) {
+    return guard([&] {
+        check_handle(sv);
+        *out = sv->last_sample_seconds;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// Host-callable launchers of the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "device.h"
+
+namespace qvd {
+
+struct DiagTargets {
+    uint32_t q[MAX_GATE_QUBITS];
+};
+
+// Device buffers of one sampling call.
+struct SampleScratch {
+    double *sums = nullptr;              // per chunk
+    double *prefix = nullptr;            // chunks + 1
+    uint64_t *bounds = nullptr;          // chunks + 1
+    double *targets_unsorted = nullptr;  // shots
+    double *targets = nullptr;           // shots, ascending
+    void *sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+};
+
+template <typename T2> void launch_init_zero(T2 *s, uint64_t len, hipStream_t st);
+template <typename T2> void launch_widen(const T2 *s, double2 *out, uint64_t count, hipStream_t st);
+template <typename T2> void launch_narrow(const double2 *in, T2 *s, uint64_t count, hipStream_t st);
+template <typename T2> void launch_scale(T2 *s, uint64_t len, double factor, hipStream_t st);
+template <typename T2> void launch_dense(T2 *s, const GateDesc &d, const T2 *mat, hipStream_t st);
+template <typename T2>
+void launch_diag(T2 *s, uint64_t len, uint32_t k, const DiagTargets &t, uint64_t cmask, uint64_t cval,
+                 const T2 *entries, hipStream_t st);
+template <typename T2> size_t stage_lds_bytes(uint32_t L);
+template <typename T2>
+void launch_stage(T2 *s, const StageDesc &d, const StageOp *ops, const T2 *coefs, unsigned grid, hipStream_t st);
+template <typename T2>
+void launch_masked_norm(const T2 *s, uint64_t len, uint64_t mask, uint64_t val, double *partial, unsigned blocks,
+                        double *out, hipStream_t st);
+uint32_t expect_segment_bits(uint32_t n);
+template <typename T2>
+void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, double *zz, hipStream_t st);
+template <typename T2>
+void launch_collapse(T2 *s, uint64_t len, uint32_t qubit, uint32_t outcome, double keep, hipStream_t st);
+size_t sort_temp_bytes(uint64_t shots);
+template <typename T2>
+hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double *uniforms, uint64_t shots,
+                   SampleScratch &w, uint64_t *out, hipStream_t st);
+
+}  // namespace qvd

@@ -1,0 +1,664 @@
+// HIP kernels of the MI355X state-vector engine (gfx950, wave64).
+//
+// State layout in HBM: 2^n interleaved complex amplitudes (float2 for
+// complex64, double2 for complex128) in logical index order; qubit q is bit q
+// of the index. Every kernel here is HBM-bound except the in-tile gate
+// arithmetic of deep stages.
+//
+// Compiled with -ffp-contract=off: fused multiply-adds appear only where
+// written (fma / fmaf), so the arithmetic is the same on every build.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace qvd {
+
+// ---------------------------------------------------------------------------
+// Complex helpers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fmul_add(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmul_add(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename T2> struct Real;
+template <> struct Real<float2> { using T = float; };
+template <> struct Real<double2> { using T = double; };
+
+// acc += w * x, as four fused multiply-adds.
+template <typename T2>
+__device__ __forceinline__ void cmac(T2 &acc, const T2 w, const T2 x) {
+    acc.x = fmul_add(w.x, x.x, acc.x);
+    acc.x = fmul_add(-w.y, x.y, acc.x);
+    acc.y = fmul_add(w.x, x.y, acc.y);
+    acc.y = fmul_add(w.y, x.x, acc.y);
+}
+
+// f * a, with the rounding of the reference's diagonal kernel
+// (re = f.re*a.re - f.im*a.im with the second product fused; im likewise,
+// src/kernels.rs:464-465).
+template <typename T2>
+__device__ __forceinline__ T2 cmul_diag(const T2 f, const T2 a) {
+    T2 r;
+    r.x = fmul_add(-f.y, a.y, f.x * a.x);
+    r.y = fmul_add(f.y, a.x, f.x * a.y);
+    return r;
+}
+
+// Insert a zero bit at each of the ascending positions pos[0..m).
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t x, const uint32_t *pos, uint32_t m) {
+    for (uint32_t j = 0; j < m; j++) {
+        const uint32_t p = pos[j];
+        const uint64_t low = x & ((uint64_t(1) << p) - 1);
+        x = ((x >> p) << (p + 1)) | low;
+    }
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation and conversion.
+// ---------------------------------------------------------------------------
+template <typename T2>
+__global__ void k_init_zero(T2 *__restrict__ s, uint64_t len) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        T2 v;
+        v.x = i == 0 ? 1 : 0;
+        v.y = 0;
+        s[i] = v;
+    }
+}
+
+template <typename T2>
+__global__ void k_widen(const T2 *__restrict__ s, double2 *__restrict__ out, uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const T2 v = s[i];
+        out[i] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
+template <typename T2>
+__global__ void k_narrow(const double2 *__restrict__ in, T2 *__restrict__ s, uint64_t count) {
+    using T = typename Real<T2>::T;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const double2 v = in[i];
+        T2 o;
+        o.x = (T)v.x;
+        o.y = (T)v.y;
+        s[i] = o;
+    }
+}
+
+template <typename T2>
+__global__ void k_scale(T2 *__restrict__ s, uint64_t len, double factor) {
+    using T = typename Real<T2>::T;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        T2 v = s[i];
+        v.x = (T)((double)v.x * factor);
+        v.y = (T)((double)v.y * factor);
+        s[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Full-state ops: one HBM pass each.
+// ---------------------------------------------------------------------------
+
+// A dense op on K targets: each thread owns one group of 2^K amplitudes
+// (the iteration index spread over the non-target, non-control bits).
+template <typename T2, int K>
+__global__ __launch_bounds__(256) void k_dense(T2 *__restrict__ s, const GateDesc d, const T2 *__restrict__ mat) {
+    constexpr int D = 1 << K;
+    extern __shared__ unsigned char smem_raw[];
+    T2 *m = reinterpret_cast<T2 *>(smem_raw);
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) m[i] = mat[i];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t it = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; it < d.iterations; it += stride) {
+        const uint64_t base = insert_zeros(it, d.ins, d.nins) | d.ctrl_val;
+        T2 a[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) a[j] = s[base + d.offsets[j]];
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            T2 acc;
+            acc.x = 0;
+            acc.y = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++) cmac(acc, m[r * D + c], a[c]);
+            s[base + d.offsets[r]] = acc;
+        }
+    }
+}
+
+// A diagonal op: amplitude i is multiplied by entries[index of its target
+// bits] where every control bit holds.
+template <typename T2>
+__global__ __launch_bounds__(256) void k_diag(T2 *__restrict__ s, uint64_t len, const uint32_t k,
+                                              const DiagTargets t, uint64_t cmask, uint64_t cval,
+                                              const T2 *__restrict__ entries) {
+    __shared__ T2 e[1 << MAX_GATE_QUBITS];
+    for (uint32_t i = threadIdx.x; i < (1u << k); i += blockDim.x) e[i] = entries[i];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        if ((i & cmask) != cval) continue;
+        uint32_t idx = 0;
+        for (uint32_t j = 0; j < k; j++) idx |= (uint32_t)((i >> t.q[j]) & 1) << j;
+        s[i] = cmul_diag(e[idx], s[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tile stages: one HBM pass applies every op of a stage.
+//
+// A workgroup loads a tile of 2^L amplitudes whose index bits are the
+// stage's tile qubits (the lowest `contig` qubits among them, so each HBM
+// access is a contiguous run), applies the stage's ops in LDS, and writes
+// the tile back. Non-tile qubits are fixed per tile: non-tile controls
+// select whether an op runs on the tile at all, and non-tile diagonal targets
+// select the entries used.
+// ---------------------------------------------------------------------------
+constexpr int STAGE_THREADS = 256;
+constexpr int STAGE_LOG_THREADS = 8;
+
+template <typename T2, int K>
+__device__ __forceinline__ void tile_dense(T2 *__restrict__ tile, const T2 *__restrict__ m, const StageOp &op,
+                                           uint32_t L) {
+    constexpr int D = 1 << K;
+    uint32_t off[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int b = 0; b < K; b++) o |= (uint32_t)((j >> b) & 1) << op.tgt[b];
+        off[j] = o;
+    }
+    const uint32_t groups = 1u << (L - op.nins);
+    for (uint32_t g = threadIdx.x; g < groups; g += STAGE_THREADS) {
+        uint32_t i0 = g;
+        for (uint32_t j = 0; j < op.nins; j++) {
+            const uint32_t p = op.ins[j];
+            i0 = ((i0 >> p) << (p + 1)) | (i0 & ((1u << p) - 1));
+        }
+        i0 |= op.lc_val;
+        T2 a[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) a[j] = tile[i0 | off[j]];
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            T2 acc;
+            acc.x = 0;
+            acc.y = 0;
+#pragma unroll
+            for (int c = 0; c < D; c++) cmac(acc, m[r * D + c], a[c]);
+            tile[i0 | off[r]] = acc;
+        }
+    }
+}
+
+template <typename T2>
+__device__ __forceinline__ void tile_diag(T2 *__restrict__ tile, const T2 *__restrict__ e, const StageOp &op,
+                                          uint32_t L, uint64_t base) {
+    uint32_t gidx = 0, lmask[MAX_GATE_QUBITS];
+    for (uint32_t j = 0; j < op.k; j++) {
+        const uint32_t t = op.tgt[j];
+        if (t & GLOBAL_BIT) {
+            gidx |= (uint32_t)((base >> (t & ~GLOBAL_BIT)) & 1) << j;
+            lmask[j] = 0;
+        } else {
+            lmask[j] = 1u << t;
+        }
+    }
+    const uint32_t tsize = 1u << L;
+    for (uint32_t i = threadIdx.x; i < tsize; i += STAGE_THREADS) {
+        if ((i & op.lc_mask) != op.lc_val) continue;
+        uint32_t idx = gidx;
+        for (uint32_t j = 0; j < op.k; j++) idx |= (uint32_t)((i & lmask[j]) != 0) << j;
+        tile[i] = cmul_diag(e[idx], tile[i]);
+    }
+}
+
+template <typename T2>
+__global__ __launch_bounds__(STAGE_THREADS) void k_stage(T2 *__restrict__ s, const StageDesc d,
+                                                         const StageOp *__restrict__ ops,
+                                                         const T2 *__restrict__ coefs) {
+    extern __shared__ unsigned char smem_raw[];
+    T2 *tile = reinterpret_cast<T2 *>(smem_raw);
+    const uint32_t L = d.L;
+    const uint32_t tsize = 1u << L;
+    T2 *mat = tile + tsize;  // coefficients of the current op (<= 16 x 16)
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lowbits = L < STAGE_LOG_THREADS ? L : STAGE_LOG_THREADS;
+    uint64_t toff = 0;
+    for (uint32_t j = 0; j < lowbits; j++) toff |= (uint64_t)((tid >> j) & 1) << d.local_pos[j];
+    const bool active = tid < tsize;
+    for (uint64_t t = blockIdx.x; t < d.num_tiles; t += gridDim.x) {
+        uint64_t base = 0;
+        for (uint32_t j = 0; j < d.nglob; j++) base |= ((t >> j) & 1) << d.glob_pos[j];
+        for (uint32_t m = 0; m < tsize; m += STAGE_THREADS) {
+            uint64_t moff = 0;
+            for (uint32_t j = lowbits; j < L; j++) moff |= (uint64_t)((m >> j) & 1) << d.local_pos[j];
+            if (active) tile[m + tid] = s[base + moff + toff];
+        }
+        __syncthreads();
+        for (uint32_t o = 0; o < d.num_ops; o++) {
+            const StageOp &op = ops[d.op_off + o];
+            if ((base & op.gc_mask) != op.gc_val) continue;
+            const uint32_t ncoef = op.kind == OP_DENSE ? (1u << (2 * op.k)) : (1u << op.k);
+            for (uint32_t c = tid; c < ncoef; c += STAGE_THREADS) mat[c] = coefs[op.coef_off + c];
+            __syncthreads();
+            if (op.kind == OP_DENSE) {
+                switch (op.k) {
+                    case 1: tile_dense<T2, 1>(tile, mat, op, L); break;
+                    case 2: tile_dense<T2, 2>(tile, mat, op, L); break;
+                    case 3: tile_dense<T2, 3>(tile, mat, op, L); break;
+                    case 4: tile_dense<T2, 4>(tile, mat, op, L); break;
+                    default: break;
+                }
+            } else {
+                tile_diag(tile, mat, op, L, base);
+            }
+            __syncthreads();
+        }
+        for (uint32_t m = 0; m < tsize; m += STAGE_THREADS) {
+            uint64_t moff = 0;
+            for (uint32_t j = lowbits; j < L; j++) moff |= (uint64_t)((m >> j) & 1) << d.local_pos[j];
+            if (active) s[base + moff + toff] = tile[m + tid];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions.
+// ---------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum of v over the block; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += scratch[w];
+    return r;
+}
+
+template <typename T2>
+__device__ __forceinline__ double prob(const T2 a) {
+    const double re = (double)a.x, im = (double)a.y;
+    return re * re + im * im;
+}
+
+// Per-block partial sums of |a_i|^2 over amplitudes with (i & mask) == val.
+template <typename T2>
+__global__ __launch_bounds__(RED_THREADS) void k_masked_norm(const T2 *__restrict__ s, uint64_t len, uint64_t mask,
+                                                             uint64_t val, double *__restrict__ partial) {
+    __shared__ double scratch[RED_THREADS / 64];
+    double acc = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride)
+        if ((i & mask) == val) acc += prob(s[i]);
+    const double r = block_sum(acc, scratch);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// Sequential sum of partial[0..m) into out[0].
+__global__ void k_sum_partials(const double *__restrict__ partial, uint32_t m, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double r = 0;
+        for (uint32_t i = 0; i < m; i++) r += partial[i];
+        out[0] = r;
+    }
+}
+
+// <Z_q> and <Z_q Z_q+1> in one pass. A block covers a segment of 2^B
+// contiguous amplitudes; per segment it writes 2B values: the total T, the
+// per-bit sums Q[q] = sum p [bit q] (q < B) and the per-pair sums
+// R[q] = sum p [bit q xor bit q+1] (q < B-1). Bits above B are constant in a
+// segment and are resolved by k_expect_combine.
+constexpr int EZ_TID_BITS = 8;   // bits from the thread index
+constexpr int EZ_MAX_M_BITS = 8; // bits from the per-thread loop
+
+template <typename T2>
+__global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ s, uint32_t B,
+                                                         double *__restrict__ out) {
+    __shared__ double scratch[4];
+    const uint32_t tb = B < EZ_TID_BITS ? B : EZ_TID_BITS;
+    const uint32_t mb = B - tb;
+    const uint32_t tid = threadIdx.x;
+    const bool active = tid < (1u << tb);
+    const uint64_t seg = blockIdx.x;
+    double T = 0, S[EZ_MAX_M_BITS], P[EZ_MAX_M_BITS];
+#pragma unroll
+    for (int j = 0; j < EZ_MAX_M_BITS; j++) S[j] = P[j] = 0;
+    if (active) {
+        const uint64_t base = (seg << B) + tid;
+        for (uint32_t m = 0; m < (1u << mb); m++) {
+            const double p = prob(s[base + ((uint64_t)m << tb)]);
+            T += p;
+#pragma unroll
+            for (int j = 0; j < EZ_MAX_M_BITS; j++) {
+                if ((uint32_t)j < mb) {
+                    const uint32_t bj = (m >> j) & 1;
+                    if (bj) S[j] += p;
+                    if ((uint32_t)j + 1 < mb && (bj ^ ((m >> (j + 1)) & 1))) P[j] += p;
+                }
+            }
+        }
+    }
+    double *o = out + seg * (2 * B);
+    // T
+    double r = block_sum(T, scratch);
+    if (tid == 0) o[0] = r;
+    // Q[q], q < tb: T times the thread's bit q; q >= tb: S[q - tb].
+    for (uint32_t q = 0; q < B; q++) {
+        double v;
+        if (q < tb) v = ((tid >> q) & 1) ? T : 0.0;
+        else {
+            v = 0;
+#pragma unroll
+            for (int j = 0; j < EZ_MAX_M_BITS; j++)
+                if ((uint32_t)j == q - tb) v = S[j];
+        }
+        r = block_sum(v, scratch);
+        if (tid == 0) o[1 + q] = r;
+    }
+    // R[q], q < B-1.
+    for (uint32_t q = 0; q + 1 < B; q++) {
+        double v;
+        if (q + 1 < tb) v = (((tid >> q) ^ (tid >> (q + 1))) & 1) ? T : 0.0;
+        else if (q + 1 == tb) {
+            // bit q from the thread, bit q+1 = loop bit 0.
+            const double s0 = mb > 0 ? S[0] : 0.0;
+            v = ((tid >> q) & 1) ? T - s0 : s0;
+        } else {
+            v = 0;
+#pragma unroll
+            for (int j = 0; j < EZ_MAX_M_BITS; j++)
+                if ((uint32_t)j == q - tb) v = P[j];
+        }
+        r = block_sum(v, scratch);
+        if (tid == 0) o[1 + B + q] = r;
+    }
+}
+
+// Combines the segment sums into z[q] = sum p (1 - 2 bit q) and
+// zz[q] = sum p (1 - 2 (bit q xor bit q+1)). One thread per output.
+__global__ void k_expect_combine(const double *__restrict__ seg_sums, uint64_t num_segs, uint32_t B,
+                                 uint32_t n, double *__restrict__ z, double *__restrict__ zz) {
+    const uint32_t w = threadIdx.x + blockIdx.x * blockDim.x;
+    if (w >= 2 * n - 1) return;
+    double tot = 0, part = 0;
+    for (uint64_t g = 0; g < num_segs; g++) {
+        const double *o = seg_sums + g * (2 * B);
+        const double T = o[0];
+        tot += T;
+        if (w < n) {
+            const uint32_t q = w;
+            if (q < B) part += o[1 + q];
+            else if ((g >> (q - B)) & 1) part += T;
+        } else {
+            const uint32_t q = w - n;
+            if (q + 1 < B) part += o[1 + B + q];
+            else if (q + 1 == B) part += ((g & 1) ? T - o[1 + q] : o[1 + q]);
+            else if (((g >> (q - B)) ^ (g >> (q + 1 - B))) & 1) part += T;
+        }
+    }
+    if (w < n) z[w] = tot - 2 * part;
+    else zz[w - n] = tot - 2 * part;
+}
+
+// ---------------------------------------------------------------------------
+// Measurement collapse (src/measure.rs:68-80): the kept half scaled by
+// `keep` in the state's precision, the other half zeroed.
+// ---------------------------------------------------------------------------
+template <typename T2>
+__global__ void k_collapse(T2 *__restrict__ s, uint64_t len, uint32_t qubit, uint32_t outcome, double keep) {
+    using T = typename Real<T2>::T;
+    const T k = (T)keep;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        T2 f;
+        if (((i >> qubit) & 1) == outcome) {
+            f.x = k;
+            f.y = 0;
+        } else {
+            f.x = 0;
+            f.y = 0;
+        }
+        s[i] = cmul_diag(f, s[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass sampling (src/measure.rs:104-174). The reference's summation
+// order is kept exactly: each chunk is summed sequentially in index order,
+// the prefix over chunks is sequential, and the walk adds probabilities one
+// by one; so one lane owns one chunk. Loads are 16-byte vectors.
+// ---------------------------------------------------------------------------
+template <typename T2>
+__global__ __launch_bounds__(256) void k_chunk_sums(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
+                                                    uint64_t num_chunks, double *__restrict__ sums) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= num_chunks) return;
+    const uint64_t start = c * chunk;
+    const uint64_t end = start + chunk < len ? start + chunk : len;
+    double acc = 0;
+    uint64_t i = start;
+    constexpr int U = 8;
+    for (; i + U <= end; i += U) {
+        T2 v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) v[j] = s[i + j];
+#pragma unroll
+        for (int j = 0; j < U; j++) acc += prob(v[j]);
+    }
+    for (; i < end; i++) acc += prob(s[i]);
+    sums[c] = acc;
+}
+
+__global__ void k_prefix(const double *__restrict__ sums, uint64_t num_chunks, double *__restrict__ prefix) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double p = 0;
+        prefix[0] = 0;
+        for (uint64_t c = 0; c < num_chunks; c++) {
+            p = p + sums[c];
+            prefix[c + 1] = p;
+        }
+    }
+}
+
+__global__ void k_targets(const double *__restrict__ u, const double *__restrict__ total_ptr, uint64_t shots,
+                          double *__restrict__ targets) {
+    const double total = *total_ptr;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < shots; i += stride)
+        targets[i] = u[i] * total;
+}
+
+// bounds[c] = number of targets < prefix[c] (partition_point).
+__global__ void k_bounds(const double *__restrict__ prefix, uint64_t num_chunks, const double *__restrict__ targets,
+                         uint64_t shots, uint64_t *__restrict__ bounds) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > num_chunks) return;
+    const double p = prefix[c];
+    uint64_t lo = 0, hi = shots;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (targets[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    bounds[c] = lo;
+}
+
+template <typename T2>
+__global__ __launch_bounds__(256) void k_walk(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
+                                              uint64_t num_chunks, const double *__restrict__ prefix,
+                                              const uint64_t *__restrict__ bounds, const double *__restrict__ targets,
+                                              uint64_t shots, uint64_t *__restrict__ out) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= num_chunks) return;
+    const uint64_t lo = bounds[c];
+    const uint64_t hi = bounds[c + 1] < shots ? bounds[c + 1] : shots;
+    if (lo >= hi) return;
+    double cumulative = prefix[c];
+    uint64_t shot = lo;
+    const uint64_t start = c * chunk;
+    const uint64_t end = start + chunk < len ? start + chunk : len;
+    uint64_t last_nonzero = start;
+    double next = targets[shot];
+    for (uint64_t i = start; i < end && shot < hi; i++) {
+        const double p = prob(s[i]);
+        if (p == 0.0) continue;
+        last_nonzero = i;
+        cumulative += p;
+        while (shot < hi && next < cumulative) {
+            out[shot] = i;
+            shot++;
+            if (shot < hi) next = targets[shot];
+        }
+    }
+    for (; shot < hi; shot++) out[shot] = last_nonzero;  // rounding leftovers
+    // indices are < len by construction: the reference's min(len - 1) is a no-op here
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(uint64_t work, unsigned threads, unsigned cap = 16384) {
+    uint64_t g = (work + threads - 1) / threads;
+    if (g == 0) g = 1;
+    return (unsigned)(g < cap ? g : cap);
+}
+
+template <typename T2>
+void launch_init_zero(T2 *s, uint64_t len, hipStream_t st) {
+    k_init_zero<T2><<<grid_for(len, 256), 256, 0, st>>>(s, len);
+}
+template <typename T2>
+void launch_widen(const T2 *s, double2 *out, uint64_t count, hipStream_t st) {
+    k_widen<T2><<<grid_for(count, 256), 256, 0, st>>>(s, out, count);
+}
+template <typename T2>
+void launch_narrow(const double2 *in, T2 *s, uint64_t count, hipStream_t st) {
+    k_narrow<T2><<<grid_for(count, 256), 256, 0, st>>>(in, s, count);
+}
+template <typename T2>
+void launch_scale(T2 *s, uint64_t len, double factor, hipStream_t st) {
+    k_scale<T2><<<grid_for(len, 256), 256, 0, st>>>(s, len, factor);
+}
+
+template <typename T2>
+void launch_dense(T2 *s, const GateDesc &d, const T2 *mat, hipStream_t st) {
+    const size_t lds = sizeof(T2) << (2 * d.k);
+    const unsigned grid = grid_for(d.iterations, 256, 65536);
+    switch (d.k) {
+        case 1: k_dense<T2, 1><<<grid, 256, lds, st>>>(s, d, mat); break;
+        case 2: k_dense<T2, 2><<<grid, 256, lds, st>>>(s, d, mat); break;
+        case 3: k_dense<T2, 3><<<grid, 256, lds, st>>>(s, d, mat); break;
+        case 4: k_dense<T2, 4><<<grid, 256, lds, st>>>(s, d, mat); break;
+        case 5: k_dense<T2, 5><<<grid, 256, lds, st>>>(s, d, mat); break;
+        case 6: k_dense<T2, 6><<<grid, 256, lds, st>>>(s, d, mat); break;
+        default: break;
+    }
+}
+
+template <typename T2>
+void launch_diag(T2 *s, uint64_t len, uint32_t k, const DiagTargets &t, uint64_t cmask, uint64_t cval,
+                 const T2 *entries, hipStream_t st) {
+    k_diag<T2><<<grid_for(len, 256, 65536), 256, 0, st>>>(s, len, k, t, cmask, cval, entries);
+}
+
+template <typename T2>
+size_t stage_lds_bytes(uint32_t L) {
+    return (sizeof(T2) << L) + sizeof(T2) * 256;
+}
+
+template <typename T2>
+void launch_stage(T2 *s, const StageDesc &d, const StageOp *ops, const T2 *coefs, unsigned grid, hipStream_t st) {
+    k_stage<T2><<<grid, STAGE_THREADS, stage_lds_bytes<T2>(d.L), st>>>(s, d, ops, coefs);
+}
+
+template <typename T2>
+void launch_masked_norm(const T2 *s, uint64_t len, uint64_t mask, uint64_t val, double *partial, unsigned blocks,
+                        double *out, hipStream_t st) {
+    k_masked_norm<T2><<<blocks, RED_THREADS, 0, st>>>(s, len, mask, val, partial);
+    k_sum_partials<<<1, 64, 0, st>>>(partial, blocks, out);
+}
+
+template <typename T2>
+void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, double *zz, hipStream_t st) {
+    const uint32_t B = n < EZ_TID_BITS + EZ_MAX_M_BITS ? n : EZ_TID_BITS + EZ_MAX_M_BITS;
+    const uint64_t segs = uint64_t(1) << (n - B);
+    k_expect_segments<T2><<<(unsigned)segs, 256, 0, st>>>(s, B, seg_sums);
+    const uint32_t outs = 2 * n - 1;
+    k_expect_combine<<<(outs + 63) / 64, 64, 0, st>>>(seg_sums, segs, B, n, z, zz);
+}
+
+uint32_t expect_segment_bits(uint32_t n) { return n < EZ_TID_BITS + EZ_MAX_M_BITS ? n : EZ_TID_BITS + EZ_MAX_M_BITS; }
+
+template <typename T2>
+void launch_collapse(T2 *s, uint64_t len, uint32_t qubit, uint32_t outcome, double keep, hipStream_t st) {
+    k_collapse<T2><<<grid_for(len, 256, 65536), 256, 0, st>>>(s, len, qubit, outcome, keep);
+}
+
+size_t sort_temp_bytes(uint64_t shots) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, (const double *)nullptr, (double *)nullptr, (int)shots) !=
+        hipSuccess)
+        return 0;
+    return bytes;
+}
+
+template <typename T2>
+hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double *uniforms, uint64_t shots,
+                         SampleScratch &w, uint64_t *out, hipStream_t st) {
+    const uint64_t nc = (len + chunk - 1) / chunk;
+    k_chunk_sums<T2><<<grid_for(nc, 256, 1u << 30), 256, 0, st>>>(s, len, chunk, nc, w.sums);
+    k_prefix<<<1, 64, 0, st>>>(w.sums, nc, w.prefix);
+    k_targets<<<grid_for(shots, 256), 256, 0, st>>>(uniforms, w.prefix + nc, shots, w.targets_unsorted);
+    size_t bytes = w.sort_temp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortKeys(w.sort_temp, bytes, w.targets_unsorted, w.targets, (int)shots, 0,
+                                                     sizeof(double) * 8, st);
+    if (e != hipSuccess) return e;
+    k_bounds<<<grid_for(nc + 1, 256, 1u << 30), 256, 0, st>>>(w.prefix, nc, w.targets, shots, w.bounds);
+    k_walk<T2><<<grid_for(nc, 256, 1u << 30), 256, 0, st>>>(s, len, chunk, nc, w.prefix, w.bounds, w.targets,
+                                                            shots, out);
+    return hipGetLastError();
+}
+
+#define QVD_INSTANTIATE(T2)                                                                                     \
+    template void launch_init_zero<T2>(T2 *, uint64_t, hipStream_t);                                            \
+    template void launch_widen<T2>(const T2 *, double2 *, uint64_t, hipStream_t);                               \
+    template void launch_narrow<T2>(const double2 *, T2 *, uint64_t, hipStream_t);                              \
+    template void launch_scale<T2>(T2 *, uint64_t, double, hipStream_t);                                        \
+    template void launch_dense<T2>(T2 *, const GateDesc &, const T2 *, hipStream_t);                            \
+    template void launch_diag<T2>(T2 *, uint64_t, uint32_t, const DiagTargets &, uint64_t, uint64_t, const T2 *, \
+                                  hipStream_t);                                                                 \
+    template size_t stage_lds_bytes<T2>(uint32_t);                                                              \
+    template void launch_stage<T2>(T2 *, const StageDesc &, const StageOp *, const T2 *, unsigned, hipStream_t); \
+    template void launch_masked_norm<T2>(const T2 *, uint64_t, uint64_t, uint64_t, double *, unsigned, double *, \
+                                         hipStream_t);                                                          \
+    template void launch_expect_z<T2>(const T2 *, uint32_t, double *, double *, double *, hipStream_t);         \
+    template void launch_collapse<T2>(T2 *, uint64_t, uint32_t, uint32_t, double, hipStream_t);                 \
+    template hipError_t launch_sample<T2>(const T2 *, uint64_t, uint64_t, const double *, uint64_t,             \
+                                          SampleScratch &, uint64_t *, hipStream_t);
+
+QVD_INSTANTIATE(float2)
+QVD_INSTANTIATE(double2)
+
+}  // namespace qvd

@@ -1,0 +1,216 @@
+// Host planning (see plan.h).
+#include "plan.h"
+
+#include <algorithm>
+
+namespace qvd {
+
+namespace {
+
+// A group being grown from a seed op (src/fusion.rs:119-172).
+struct Group {
+    size_t seed;
+    std::vector<size_t> members;
+    std::vector<uint32_t> qubits;
+
+    template <typename W>
+    void grow(const std::vector<Op> &ops, std::vector<char> &done, size_t max_qubits, bool connected_only,
+              const W &width) {
+        std::vector<uint32_t> blocked;
+        for (size_t i = seed + 1; i < ops.size(); i++) {
+            if (done[i]) continue;
+            if (connected_only) {
+                bool all = true;
+                for (uint32_t q : qubits)
+                    if (!contains(blocked, q)) {
+                        all = false;
+                        break;
+                    }
+                if (all) break;
+            }
+            const std::vector<uint32_t> oq = ops[i].qubits();
+            bool touches = false;
+            for (uint32_t q : oq) touches |= contains(blocked, q);
+            std::vector<uint32_t> fresh;
+            for (uint32_t q : oq)
+                if (!contains(qubits, q)) fresh.push_back(q);
+            const bool connected = fresh.size() < oq.size();
+            const bool fits = width(ops[i]) <= max_qubits && qubits.size() + fresh.size() <= max_qubits;
+            if (!touches && fits && (connected || !connected_only)) {
+                done[i] = 1;
+                members.push_back(i);
+                qubits.insert(qubits.end(), fresh.begin(), fresh.end());
+            } else {
+                for (uint32_t q : oq)
+                    if (!contains(blocked, q)) blocked.push_back(q);
+            }
+        }
+        std::sort(members.begin(), members.end());
+    }
+};
+
+}  // namespace
+
+template <typename F>
+std::vector<Op> fuse_where(const std::vector<Op> &ops, size_t max_qubits, const F &fusable) {
+    auto width = [&](const Op &op) { return fusable(op) ? op.width() : SIZE_MAX; };
+    std::vector<char> done(ops.size(), 0);
+    std::vector<Op> out;
+    for (size_t seed = 0; seed < ops.size(); seed++) {
+        if (done[seed]) continue;
+        done[seed] = 1;
+        Group g{seed, {seed}, ops[seed].qubits()};
+        if (width(ops[seed]) <= max_qubits) {
+            g.grow(ops, done, max_qubits, true, width);
+            g.grow(ops, done, max_qubits, false, width);
+        }
+        if (g.members.size() == 1) {
+            out.push_back(ops[seed]);
+            continue;
+        }
+        std::sort(g.qubits.begin(), g.qubits.end());
+        HMatrix m = HMatrix::identity((uint32_t)g.qubits.size());
+        for (size_t i : g.members) {
+            std::vector<uint32_t> oq;
+            HMatrix om;
+            ops[i].full(oq, om);
+            m = om.embed(oq, g.qubits).mul(m);
+        }
+        Op f;
+        f.targets = g.qubits;
+        f.m = std::move(m);
+        f.diagonal = f.m.is_diagonal();
+        out.push_back(std::move(f));
+    }
+    return out;
+}
+
+std::vector<Op> fuse(const std::vector<Op> &ops, size_t max_qubits) {
+    return fuse_where(ops, max_qubits, [](const Op &) { return true; });
+}
+
+namespace {
+
+// Take every pending op that can run now and whose qubits no earlier
+// pending op blocks (src/blocking.rs:157-179).
+template <typename F>
+std::vector<size_t> extract(const std::vector<Op> &ops, const std::vector<char> &done, F &&can_run) {
+    std::vector<size_t> taken;
+    std::vector<uint32_t> blocked;
+    for (size_t i = 0; i < ops.size(); i++) {
+        if (done[i]) continue;
+        const std::vector<uint32_t> oq = ops[i].qubits();
+        bool touches = false;
+        for (uint32_t q : oq) touches |= contains(blocked, q);
+        if (!touches && can_run(ops[i])) {
+            taken.push_back(i);
+        } else {
+            for (uint32_t q : oq)
+                if (!contains(blocked, q)) blocked.push_back(q);
+        }
+    }
+    return taken;
+}
+
+}  // namespace
+
+std::vector<Step> plan_unblocked(const std::vector<Op> &ops, uint32_t max_fused) {
+    std::vector<Step> steps;
+    for (Op &op : fuse(ops, std::max<uint32_t>(max_fused, 1))) {
+        Step s;
+        s.is_stage = false;
+        s.op = std::move(op);
+        steps.push_back(std::move(s));
+    }
+    return steps;
+}
+
+std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p) {
+    const uint32_t n = p.num_qubits;
+    const size_t max_fused = std::max<uint32_t>(p.max_fused, 1);
+    std::vector<Step> steps;
+    if (ops.empty()) return steps;
+    const uint32_t L = std::min(p.tile_bits, n);
+    const uint32_t contig = std::min(p.contig, L);
+    std::vector<char> is_local(n, 0);
+    std::vector<uint32_t> local;
+    for (uint32_t q = 0; q < L; q++) {
+        local.push_back(q);
+        is_local[q] = 1;
+    }
+    // A dense op runs inside a tile when its targets are tile qubits and it
+    // is narrow enough for the in-tile kernel; a diagonal op always runs
+    // (its non-tile targets and every non-tile control are fixed per tile).
+    auto runs_locally = [&](const Op &op) {
+        if (op.diagonal) return true;
+        if (op.targets.size() > p.max_stage_op) return false;
+        for (uint32_t q : op.targets)
+            if (!is_local[q]) return false;
+        return true;
+    };
+    std::vector<char> done(ops.size(), 0);
+    size_t remaining = ops.size();
+    while (remaining > 0) {
+        std::vector<size_t> taken = extract(ops, done, runs_locally);
+        if (!taken.empty()) {
+            std::vector<Op> stage_ops;
+            for (size_t i : taken) {
+                done[i] = 1;
+                stage_ops.push_back(ops[i]);
+            }
+            remaining -= taken.size();
+            // Fuse only ops on tile qubits: merging a non-tile control or
+            // diagonal target into a dense matrix would need it in the tile.
+            std::vector<Op> fused = fuse_where(stage_ops, std::min<size_t>(max_fused, p.max_stage_op), [&](const Op &op) {
+                for (uint32_t q : op.qubits())
+                    if (!is_local[q]) return false;
+                return true;
+            });
+            Step s;
+            s.stage.local = local;
+            s.stage.ops = std::move(fused);
+            steps.push_back(std::move(s));
+        }
+        if (remaining == 0) break;
+        // Lookahead (src/blocking.rs:181-226): the qubits the coming ops
+        // need, within the tile's free capacity.
+        const uint32_t capacity = L - contig;
+        std::vector<uint32_t> wanted;
+        extract(ops, done, [&](const Op &op) {
+            if (op.diagonal) return true;
+            if (op.targets.size() > p.max_stage_op) return false;
+            std::vector<uint32_t> fresh;
+            for (uint32_t q : op.targets)
+                if (q >= contig && !contains(wanted, q)) fresh.push_back(q);
+            if (wanted.size() + fresh.size() > capacity) return false;
+            wanted.insert(wanted.end(), fresh.begin(), fresh.end());
+            return true;
+        });
+        std::vector<uint32_t> next;
+        for (uint32_t q = 0; q < contig; q++) next.push_back(q);
+        for (uint32_t q : wanted) next.push_back(q);
+        for (uint32_t q = contig; q < n && next.size() < L; q++)
+            if (!contains(next, q)) next.push_back(q);
+        std::sort(next.begin(), next.end());
+        bool same = next == local;
+        if (same) {
+            // Nothing new can be brought in (an op wider than the tile's
+            // free capacity): one HBM pass for the first pending op.
+            size_t i = 0;
+            while (done[i]) i++;
+            done[i] = 1;
+            remaining--;
+            Step s;
+            s.is_stage = false;
+            s.op = ops[i];
+            steps.push_back(std::move(s));
+            continue;
+        }
+        local = next;
+        std::fill(is_local.begin(), is_local.end(), 0);
+        for (uint32_t q : local) is_local[q] = 1;
+    }
+    return steps;
+}
+
+}  // namespace qvd

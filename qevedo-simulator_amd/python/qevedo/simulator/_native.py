@@ -1,0 +1,286 @@
+"""ctypes binding of ``libqvd_hip.so`` (the C ABI in ``include/qvd_hip.h``).
+
+This module is the only way the Python package reaches the engine; every
+computation runs in the HIP library on the GPU. If the library is missing or
+no GPU is visible, calls raise ``NativeError`` instead of computing anything
+another way.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", "..", ".."))
+LIB_PATH = os.environ.get("QVD_LIB", os.path.join(_PKG_ROOT, "lib", "libqvd_hip.so"))
+
+QVD_OK, QVD_EINVAL, QVD_ENOMEM, QVD_EHIP, QVD_EUNSUPPORTED = 0, -1, -2, -3, -4
+C64, C128 = 0, 1
+PLANNER_HEURISTIC, PLANNER_LEARNED = 0, 1
+
+# Every symbol include/qvd_hip.h declares.
+EXPORTS = (
+    "qvd_last_error", "qvd_device_info", "qvd_sv_create", "qvd_sv_destroy", "qvd_sv_num_qubits",
+    "qvd_sv_precision", "qvd_sv_set_zero", "qvd_sv_set_amplitudes", "qvd_sv_get_amplitudes", "qvd_sv_apply",
+    "qvd_sv_apply_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
+    "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_last_sample_seconds",
+    "qvd_gate_matrix", "qvd_gate_lower",
+)
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class Op(C.Structure):
+    _fields_ = [
+        ("targets", C.POINTER(C.c_uint32)),
+        ("num_targets", C.c_uint32),
+        ("controls", C.POINTER(C.c_uint32)),
+        ("control_values", C.POINTER(C.c_uint8)),
+        ("num_controls", C.c_uint32),
+        ("matrix", C.POINTER(C.c_double)),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("max_fused_qubits", C.c_uint32),
+        ("tile_bits", C.c_int32),
+        ("planner", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("gates", C.c_uint64),
+        ("kernels", C.c_uint64),
+        ("passes", C.c_uint64),
+        ("stages", C.c_uint64),
+        ("gate_seconds", C.c_double),
+        ("measure_seconds", C.c_double),
+        ("plan_seconds", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine; raises NativeError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(QVD_EHIP, f"{LIB_PATH} is not built (run `make -C qevedo-simulator_amd`)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        u32, u64, i32, d, vp = C.c_uint32, C.c_uint64, C.c_int32, C.c_double, C.c_void_p
+        L.qvd_last_error.restype = C.c_char_p
+        L.qvd_device_info.argtypes = [P(C.c_int), C.c_char_p, u32]
+        L.qvd_sv_create.argtypes = [u32, C.c_int, P(vp)]
+        L.qvd_sv_destroy.argtypes = [vp]
+        L.qvd_sv_num_qubits.argtypes = [vp, P(u32)]
+        L.qvd_sv_precision.argtypes = [vp, P(C.c_int)]
+        L.qvd_sv_set_zero.argtypes = [vp]
+        L.qvd_sv_set_amplitudes.argtypes = [vp, P(d), u64]
+        L.qvd_sv_get_amplitudes.argtypes = [vp, P(d), u64]
+        L.qvd_sv_apply.argtypes = [vp, P(d), P(u32), u32, P(u32), P(C.c_uint8), u32]
+        L.qvd_sv_apply_ops.argtypes = [vp, P(Op), u64, P(Options), P(Stats)]
+        L.qvd_sv_norm_sqr.argtypes = [vp, P(d)]
+        L.qvd_sv_scale.argtypes = [vp, d]
+        L.qvd_sv_probability_one.argtypes = [vp, u32, P(d)]
+        L.qvd_sv_probabilities.argtypes = [vp, P(d), u64]
+        L.qvd_sv_expect_z.argtypes = [vp, P(d), P(d)]
+        L.qvd_sv_measure.argtypes = [vp, u32, d, P(C.c_uint8)]
+        L.qvd_sv_reset.argtypes = [vp, u32, d]
+        L.qvd_sv_sample.argtypes = [vp, P(d), u64, P(u64)]
+        L.qvd_sv_last_sample_seconds.argtypes = [vp, P(d)]
+        L.qvd_gate_matrix.argtypes = [C.c_char_p, P(d), u32, P(d), u32, P(u32)]
+        L.qvd_gate_lower.argtypes = [C.c_char_p, P(d), u32, P(u32), P(u32), P(d), u32]
+        for name in EXPORTS:
+            getattr(L, name)  # fail early on a missing symbol
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != QVD_OK:
+        raise NativeError(rc, lib().qvd_last_error().decode())
+
+
+def _p(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def device_info() -> tuple:
+    n = C.c_int()
+    arch = C.create_string_buffer(64)
+    _check(lib().qvd_device_info(C.byref(n), arch, 64))
+    return n.value, arch.value.decode()
+
+
+def gate_matrix(name: str, params=()) -> np.ndarray:
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros(2 * 64, dtype=np.float64)
+    k = C.c_uint32()
+    _check(lib().qvd_gate_matrix(name.encode(), _p(p, C.c_double), p.size, _p(out, C.c_double), 64, C.byref(k)))
+    dim = 1 << k.value
+    return out[: 2 * dim * dim].view(np.complex128).reshape(dim, dim).copy()
+
+
+def gate_lower(name: str, params=()) -> tuple:
+    """(number of leading controls, base matrix) of a named gate."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros(2 * 64, dtype=np.float64)
+    nc, nt = C.c_uint32(), C.c_uint32()
+    _check(lib().qvd_gate_lower(name.encode(), _p(p, C.c_double), p.size, C.byref(nc), C.byref(nt), _p(out, C.c_double), 64))
+    dim = 1 << nt.value
+    return nc.value, out[: 2 * dim * dim].view(np.complex128).reshape(dim, dim).copy()
+
+
+def lower_gates(gates) -> list:
+    """gate_op over a list of (name, qubits, params): [(targets, controls, matrix)]."""
+    cache = {}
+    ops = []
+    for name, qubits, params in gates:
+        key = (name, tuple(params))
+        if key not in cache:
+            cache[key] = gate_lower(name, params)
+        c, m = cache[key]
+        qubits = list(qubits)
+        ops.append((qubits[c:], [(q, True) for q in qubits[:c]], m))
+    return ops
+
+
+class OpArray:
+    """A qvd_op array plus the NumPy buffers it points into."""
+
+    def __init__(self, ops):
+        self.keep = []
+        self.n = len(ops)
+        self.arr = (Op * max(self.n, 1))()
+        for i, (targets, controls, matrix) in enumerate(ops):
+            t = np.ascontiguousarray(targets, dtype=np.uint32)
+            cq = np.ascontiguousarray([c[0] for c in controls], dtype=np.uint32)
+            cv = np.ascontiguousarray([1 if c[1] else 0 for c in controls], dtype=np.uint8)
+            m = np.ascontiguousarray(matrix, dtype=np.complex128)
+            if m.shape != (1 << t.size, 1 << t.size):
+                raise ValueError(f"a gate on {t.size} qubits needs a {1 << t.size}x{1 << t.size} matrix")
+            self.keep += [t, cq, cv, m]
+            self.arr[i] = Op(_p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size, _p(m.view(np.float64), C.c_double))
+
+
+class StateVector:
+    """A state of `num_qubits` qubits in HBM (StateVector, src/state.rs).
+
+    precision: "single" (complex64) or "double" (complex128).
+    """
+
+    def __init__(self, num_qubits: int, precision: str = "single"):
+        prec = {"single": C64, "double": C128}.get(precision)
+        if prec is None:
+            raise ValueError('precision must be "single" or "double"')
+        self.num_qubits = int(num_qubits)
+        self.precision = precision
+        h = C.c_void_p()
+        _check(lib().qvd_sv_create(self.num_qubits, prec, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().qvd_sv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def size(self) -> int:
+        return 1 << self.num_qubits
+
+    def set_zero(self) -> None:
+        _check(lib().qvd_sv_set_zero(self.h))
+
+    def set_amplitudes(self, amps) -> None:
+        a = np.ascontiguousarray(amps, dtype=np.complex128)
+        _check(lib().qvd_sv_set_amplitudes(self.h, _p(a.view(np.float64), C.c_double), a.size))
+
+    def amplitudes(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=np.complex128)
+        _check(lib().qvd_sv_get_amplitudes(self.h, _p(out.view(np.float64), C.c_double), out.size))
+        return out
+
+    def apply(self, matrix, targets, controls=()) -> None:
+        m = np.ascontiguousarray(matrix, dtype=np.complex128)
+        t = np.ascontiguousarray(targets, dtype=np.uint32)
+        cq = np.ascontiguousarray([c[0] for c in controls], dtype=np.uint32)
+        cv = np.ascontiguousarray([1 if c[1] else 0 for c in controls], dtype=np.uint8)
+        _check(lib().qvd_sv_apply(self.h, _p(m.view(np.float64), C.c_double), _p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size))
+
+    def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC) -> dict:
+        """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default."""
+        arr = ops if isinstance(ops, OpArray) else OpArray(ops)
+        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0)
+        st = Stats()
+        _check(lib().qvd_sv_apply_ops(self.h, arr.arr, arr.n, C.byref(opt), C.byref(st)))
+        return st.as_dict()
+
+    def norm_sqr(self) -> float:
+        v = C.c_double()
+        _check(lib().qvd_sv_norm_sqr(self.h, C.byref(v)))
+        return v.value
+
+    def scale(self, factor: float) -> None:
+        _check(lib().qvd_sv_scale(self.h, factor))
+
+    def probability_one(self, qubit: int) -> float:
+        v = C.c_double()
+        _check(lib().qvd_sv_probability_one(self.h, qubit, C.byref(v)))
+        return v.value
+
+    def probabilities(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=np.float64)
+        _check(lib().qvd_sv_probabilities(self.h, _p(out, C.c_double), out.size))
+        return out
+
+    def expect_z(self) -> tuple:
+        n = self.num_qubits
+        z = np.zeros(max(n, 1), dtype=np.float64)
+        zz = np.zeros(max(n - 1, 1), dtype=np.float64)
+        _check(lib().qvd_sv_expect_z(self.h, _p(z, C.c_double), _p(zz, C.c_double)))
+        return z[:n], zz[: max(n - 1, 0)]
+
+    def measure(self, qubit: int, uniform: float) -> int:
+        o = C.c_uint8()
+        _check(lib().qvd_sv_measure(self.h, qubit, uniform, C.byref(o)))
+        return o.value
+
+    def reset(self, qubit: int, uniform: float) -> None:
+        _check(lib().qvd_sv_reset(self.h, qubit, uniform))
+
+    def sample(self, uniforms) -> np.ndarray:
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        out = np.empty(u.size, dtype=np.uint64)
+        _check(lib().qvd_sv_sample(self.h, _p(u, C.c_double), u.size, _p(out, C.c_uint64)))
+        return out
+
+    def last_sample_seconds(self) -> float:
+        v = C.c_double()
+        _check(lib().qvd_sv_last_sample_seconds(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,320 @@
+"""Parity of the HIP engine (through its C ABI) with the oracle.
+
+Mirrors the reference's integration tests (tests/kernels.rs,
+tests/simulator.rs) at the same shapes: random single ops against the FP64
+applier, circuits at every fusion width, tiny tiles that force many stages,
+non-tile controls and diagonals, sampling bit-exact against the restated
+two-pass sampler on the same amplitudes and uniforms.
+Tolerances: complex128 1e-12 per op / 1e-11 per circuit, complex64 2e-6 per
+op / 1e-5 per circuit (max-abs), as the reference's tests use.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import (lower_with_oracle, max_distance, oracle_statevector, random_diagonal, random_qubits,
+                     random_state, random_unitary, rel_l2, rng)
+from oracle import oracle
+from qevedo.simulator import NativeError, StateVector, library, lower_gates
+from qevedo.simulator import _native
+
+pytestmark = pytest.mark.gpu
+
+PREC = [("single", 2e-6, 1e-5), ("double", 1e-12, 1e-11)]
+
+
+def _note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+@pytest.mark.parametrize("precision,tol,_", PREC)
+def test_single_ops_match_reference(precision, tol, _):
+    """tests/kernels.rs:12-62: 400 random ops, n in [1,11], k in [1,6],
+    up to two controls, 30% diagonal."""
+    _note(f"single ops {precision}")
+    r = rng(1 if precision == "double" else 2)
+    for case in range(400):
+        n = int(r.integers(1, 12))
+        k = int(r.integers(1, min(n, 6) + 1))
+        qubits = random_qubits(r, n, k)
+        rest = [q for q in range(n) if q not in qubits]
+        if not rest or k >= 6 or r.random() < 0.5:
+            controls = []
+        else:
+            c = int(r.integers(1, min(len(rest), 2, 6 - k) + 1))
+            controls = [(q, bool(r.random() < 0.7)) for q in rest[:c]]
+        diagonal = r.random() < 0.3
+        m = random_diagonal(r, k) if diagonal else random_unitary(r, k)
+        initial = random_state(r, n)
+        expected = initial.copy()
+        bits = sum((1 if v else 0) << i for i, (_, v) in enumerate(controls))
+        full = oracle.matrix_controlled(m, len(controls), bits) if controls else m
+        oracle.reference_apply(expected, full, [q for q, _ in controls] + qubits)
+        with StateVector(n, precision) as s:
+            s.set_amplitudes(initial)
+            s.apply(m, qubits, controls)
+            d = max_distance(s.amplitudes(), expected)
+        assert d < tol, f"case {case}: n={n} q={qubits} c={controls} diag={diagonal}: {d}"
+
+
+def test_large_state_every_width():
+    # tests/kernels.rs:74-92, extended to k = 6.
+    _note("large state")
+    r = rng(3)
+    n = 18
+    initial = random_state(r, n)
+    for k in range(1, 7):
+        q = random_qubits(r, n, k)
+        m = random_unitary(r, k)
+        expected = initial.copy()
+        oracle.reference_apply(expected, m, q)
+        with StateVector(n, "double") as s:
+            s.set_amplitudes(initial)
+            s.apply(m, q)
+            assert max_distance(s.amplitudes(), expected) < 1e-12, k
+
+
+@pytest.mark.parametrize("tile_bits", [None, 0])
+def test_circuits_every_fusion_width(tile_bits):
+    # tests/simulator.rs:8-33, with and without tiles.
+    _note(f"fusion widths tile_bits={tile_bits}")
+    for seed in range(40):
+        n = 1 + seed % 10
+        gates = library.random_gate_circuit(n, 60, seed)
+        expected = oracle_statevector(n, gates)
+        ops = lower_gates(gates)
+        for width in range(1, 7):
+            with StateVector(n, "double") as s:
+                st = s.apply_ops(ops, width, tile_bits)
+                assert max_distance(s.amplitudes(), expected) < 1e-11, (seed, width)
+                assert st["kernels"] <= st["gates"]
+            with StateVector(n, "single") as s:
+                s.apply_ops(ops, width, tile_bits)
+                assert max_distance(s.amplitudes(), expected) < 1e-5, (seed, width)
+
+
+def test_tiny_tiles_force_stages():
+    """tests/simulator.rs:164-193: tiles of 2..6 qubits on 8..14-qubit states
+    force many stages, non-tile controls and diagonals, and full-state ops."""
+    _note("tiny tiles")
+    for seed in range(30):
+        n = 8 + seed % 7
+        gates = library.random_gate_circuit(n, 120, 100 + seed)
+        expected = oracle_statevector(n, gates)
+        ops = lower_gates(gates)
+        for tile in (2, 3, 4, 6):
+            for width in (1, 3, 4):
+                with StateVector(n, "double") as s:
+                    st = s.apply_ops(ops, width, tile)
+                    assert max_distance(s.amplitudes(), expected) < 1e-11, (seed, tile, width)
+                    assert st["passes"] > 0
+                with StateVector(n, "single") as s:
+                    s.apply_ops(ops, width, tile)
+                    assert max_distance(s.amplitudes(), expected) < 1e-5, (seed, tile, width)
+
+
+def test_library_circuits():
+    # tests/simulator.rs:35-57
+    _note("library circuits")
+    for n, gates in [(9, library.ghz(9)), (9, library.qft(9)), (10, library.random_circuit(10, 12, 7))]:
+        expected = oracle_statevector(n, gates)
+        with StateVector(n, "double") as s:
+            st = s.apply_ops(lower_gates(gates), 4, 0)
+            assert max_distance(s.amplitudes(), expected) < 1e-12
+            assert st["kernels"] < st["gates"]
+    gates = library.random_circuit(20, 20, 1)
+    with StateVector(20, "single") as s:
+        st = s.apply_ops(lower_gates(gates), 4, 0)
+    assert st["gates"] > 500 and st["kernels"] * 4 < st["gates"]
+
+
+@pytest.mark.parametrize("n", [20, 22])
+def test_random_circuit_against_blocked_fp64(n):
+    """The benchmark circuit family at 20-22 qubits: complex64 against the
+    oracle's FP64 engine (itself pinned against the naive applier)."""
+    _note(f"random circuit n={n}")
+    gates = library.random_circuit(n, 20, 42)
+    with oracle.OracleState(n, "double") as o:
+        o.apply_ops(lower_with_oracle(gates), 4, 14)
+        want = o.amplitudes()
+    with StateVector(n, "single") as s:
+        s.apply_ops(lower_gates(gates), 4, 0)
+        got = s.amplitudes()
+    assert rel_l2(got, want) <= 1e-5
+    with StateVector(n, "double") as s:
+        s.apply_ops(lower_gates(gates), 4, 0)
+        assert max_distance(s.amplitudes(), want) < 1e-11
+
+
+def test_qft_closed_form():
+    _note("qft closed form")
+    for n in (12, 20):
+        with StateVector(n, "single") as s:
+            s.apply_ops(lower_gates(library.qft(n)), 4, 0)
+            assert max_distance(s.amplitudes(), np.full(1 << n, 2 ** (-n / 2))) < 1e-6
+
+
+def test_tile_choice_and_determinism():
+    """The same circuit twice gives bit-identical amplitudes; other tile
+    sizes stay within the FP64 bound (the analogue of the thread-count
+    reproducibility test, python/tests/test_simulator.py:167-176)."""
+    _note("determinism")
+    n = 16
+    gates = library.random_circuit(n, 12, 3)
+    ops = lower_gates(gates)
+    runs = []
+    for tile in (0, 0, 8, 11, 13):
+        with StateVector(n, "single") as s:
+            s.apply_ops(ops, 4, tile)
+            runs.append(s.amplitudes())
+    assert np.array_equal(runs[0], runs[1])
+    want = oracle_statevector(n, gates)
+    for a in runs:
+        assert max_distance(a, want) < 1e-5
+
+
+def _oracle_samples(amps, precision, uniforms):
+    n = int(round(math.log2(amps.size)))
+    with oracle.OracleState(n, precision) as o:
+        o.set_amplitudes(amps)
+        return o.sample(uniforms)
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("n", [3, 15, 17, 19])
+def test_sampling_bit_exact(precision, n):
+    """measure.rs:104-174 on the same amplitudes and uniforms: identical
+    outcome vectors (multi-chunk states, zero runs, boundary uniforms)."""
+    _note(f"sampling {precision} n={n}")
+    r = rng(100 + n)
+    a = random_state(r, n)
+    a[r.integers(0, 1 << n, max(1, (1 << n) // 16))] = 0
+    a[: (1 << n) // 8] = 0
+    with StateVector(n, precision) as s:
+        s.set_amplitudes(a)
+        stored = s.amplitudes()  # the values the GPU holds, in their precision
+        u = r.random(20000)
+        u[:4] = [0.0, 0.25, 0.5, 1 - 2 ** -53]
+        got = s.sample(u)
+    want = _oracle_samples(stored, precision, u)
+    assert np.array_equal(got, want)
+
+
+def test_sampling_after_circuit_matches_oracle():
+    _note("sampling after circuit")
+    n = 18
+    gates = library.random_circuit(n, 10, 5)
+    with StateVector(n, "single") as s:
+        s.apply_ops(lower_gates(gates), 4, 0)
+        amps = s.amplitudes()
+        u = rng(9).random(100_000)
+        got = s.sample(u)
+    assert np.array_equal(got, _oracle_samples(amps, "single", u))
+
+
+def test_bell_and_distribution_sampling():
+    # tests/simulator.rs:59-111
+    _note("bell sampling")
+    with StateVector(2, "single") as s:
+        s.apply_ops(lower_gates([("h", (0,), ()), ("cx", (0, 1), ())]))
+        out = s.sample(rng(5).random(100_000))
+    counts = np.bincount(out.astype(np.int64), minlength=4)
+    assert counts[1] == 0 and counts[2] == 0 and abs(counts[0] - 50_000) < 800
+    gates = [("ry", (0,), (1.0,)), ("ry", (1,), (2.0,)), ("cx", (1, 2), ())]
+    shots = 200_000
+    with StateVector(3, "double") as s:
+        s.apply_ops(lower_gates(gates))
+        out = s.sample(rng(9).random(shots))
+    p = np.abs(oracle_statevector(3, gates)) ** 2
+    obs = np.bincount(out.astype(np.int64), minlength=8) / shots
+    for i in range(8):
+        assert abs(obs[i] - p[i]) <= 5 * math.sqrt(p[i] * (1 - p[i]) / shots) + 1e-9
+
+
+def test_reductions():
+    _note("reductions")
+    r = rng(4)
+    for n, precision in [(1, "double"), (5, "double"), (16, "single"), (20, "double")]:
+        a = random_state(r, n)
+        with StateVector(n, precision) as s:
+            s.set_amplitudes(a)
+            stored = s.amplitudes()
+            p = np.abs(stored) ** 2
+            idx = np.arange(1 << n)
+            assert abs(s.norm_sqr() - p.sum()) < 1e-12
+            for q in range(n):
+                assert abs(s.probability_one(q) - p[((idx >> q) & 1) == 1].sum()) < 1e-12
+            z, zz = s.expect_z()
+            for q in range(n):
+                bit = (idx >> q) & 1
+                assert abs(z[q] - np.sum(p * (1 - 2 * bit))) < 1e-12
+                if q + 1 < n:
+                    par = bit ^ ((idx >> (q + 1)) & 1)
+                    assert abs(zz[q] - np.sum(p * (1 - 2 * par))) < 1e-12
+            assert max_distance(s.probabilities(), p) < 1e-15
+            s.scale(2.0)
+            assert abs(s.norm_sqr() - 4 * p.sum()) < 1e-11
+
+
+def test_measure_and_reset_follow_the_uniform():
+    """measure.rs:68-87 with caller uniforms: outcome = u < P(1), collapse
+    and renormalise; compared with the oracle step by step."""
+    _note("measure/reset")
+    r = rng(12)
+    n = 10
+    gates = library.random_circuit(n, 6, 2)
+    for precision in ("single", "double"):
+        with StateVector(n, precision) as s, oracle.OracleState(n, precision) as o:
+            s.apply_ops(lower_gates(gates))
+            o.set_amplitudes(s.amplitudes())
+            for step in range(6):
+                q = int(r.integers(0, n))
+                u = float(r.random())
+                p1 = o.probability_one(q)
+                if abs(u - p1) < 1e-6:
+                    continue
+                if step % 2:
+                    s.reset(q, u)
+                    o.reset(q, u)
+                else:
+                    assert s.measure(q, u) == o.measure(q, u)
+                tol = 2e-6 if precision == "single" else 1e-12
+                assert max_distance(s.amplitudes(), o.amplitudes()) < tol
+    # tests/simulator.rs:113-139: x; measure; reset; measure; h; measure
+    counts = {}
+    rr = rng(3)
+    for _ in range(2000):
+        with StateVector(1, "double") as s:
+            c = [0, 0, 0]
+            s.apply(_native.gate_matrix("x"), [0])
+            c[0] = s.measure(0, rr.random())
+            s.reset(0, rr.random())
+            c[1] = s.measure(0, rr.random())
+            s.apply(_native.gate_matrix("h"), [0])
+            c[2] = s.measure(0, rr.random())
+        key = "".join(str(b) for b in reversed(c))
+        counts[key] = counts.get(key, 0) + 1
+    assert sorted(counts) == ["001", "101"] and abs(counts["101"] - 1000) < 200
+
+
+def test_errors_are_status_codes():
+    _note("errors")
+    with StateVector(4) as s:
+        with pytest.raises(NativeError) as e:
+            s.apply(np.eye(2), [4])
+        assert e.value.code == _native.QVD_EINVAL and "out of range" in str(e.value)
+        with pytest.raises(NativeError) as e:
+            s.apply(np.eye(4), [1, 1])
+        assert e.value.code == _native.QVD_EINVAL
+        with pytest.raises(NativeError) as e:
+            s.set_amplitudes(np.zeros(8))
+        assert e.value.code == _native.QVD_EINVAL
+    with StateVector(8) as s:
+        with pytest.raises(NativeError) as e:
+            s.apply(np.eye(128), list(range(7)))
+        assert e.value.code == _native.QVD_EUNSUPPORTED
+    with pytest.raises(NativeError) as e:
+        StateVector(35)
+    assert e.value.code == _native.QVD_EINVAL
