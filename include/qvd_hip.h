@@ -64,8 +64,12 @@ typedef struct qvd_options {
     uint32_t max_fused_qubits; /* largest fused gate; 1 disables fusion (default 4) */
     int32_t tile_bits;         /* qubits per LDS-resident tile; <0 = no blocking, 0 = default */
     uint32_t planner;          /* QVD_PLANNER_* */
-    uint32_t reserved;
+    uint32_t flags;            /* QVD_FLAG_* */
 } qvd_options;
+
+/* qvd_options.flags: time every kernel launch with HIP events on the
+ * handle's stream and report the sums in qvd_stats. */
+#define QVD_FLAG_TIMING 1u
 
 /* What a run did: simulator::Stats (src/simulator.rs:70-93). */
 typedef struct qvd_stats {
@@ -75,7 +79,9 @@ typedef struct qvd_stats {
     uint64_t stages;  /* tile stages launched */
     double gate_seconds;
     double measure_seconds;
-    double plan_seconds; /* host planning inside gate_seconds */
+    double plan_seconds;          /* host planning inside gate_seconds */
+    double stage_kernel_seconds;  /* sum of tile-stage kernel durations (QVD_FLAG_TIMING) */
+    double other_kernel_seconds;  /* sum of full-state op kernel durations (QVD_FLAG_TIMING) */
 } qvd_stats;
 
 const char *qvd_last_error(void);
@@ -131,6 +137,13 @@ int qvd_sv_reset(qvd_sv *sv, uint32_t qubit, double uniform);
  * i-th smallest target; bit-exact with the reference's algorithm on the same
  * amplitudes and uniforms. */
 int qvd_sv_sample(const qvd_sv *sv, const double *uniforms, uint64_t shots, uint64_t *out);
+
+/* Distance between two states of the same size (any precisions), in FP64
+ * on the device: max_abs = max |a_i - b_i|, diff_sq = sum |a_i - b_i|^2,
+ * ref_sq = sum |b_i|^2 (relative L2 error = sqrt(diff_sq / ref_sq)).
+ * New in this engine: it checks a complex64 run against a complex128 run
+ * without copying either state to the host. */
+int qvd_sv_compare(const qvd_sv *a, const qvd_sv *b, double *max_abs, double *diff_sq, double *ref_sq);
 
 /* Sampling statistics of the last qvd_sv_sample on this handle (seconds). */
 int qvd_sv_last_sample_seconds(const qvd_sv *sv, double *out);

@@ -130,6 +130,27 @@ def gate_controls(name: str) -> int:
     return c.value
 
 
+def lower(gates) -> list:
+    """gate_op (simulator.rs:106-119) over the oracle's gate table for a list
+    of (name, qubits, params): controls stay separate, the matrix is the
+    base gate's."""
+    ops = []
+    cache = {}
+    for name, qubits, params in gates:
+        key = (name, tuple(params))
+        if key not in cache:
+            full = gate_matrix(name, params)
+            c = gate_controls(name)
+            if c:
+                mask = (1 << c) - 1
+                idx = [(i << c) | mask for i in range(full.shape[0] >> c)]
+                full = full[np.ix_(idx, idx)]
+            cache[key] = (c, full)
+        c, m = cache[key]
+        ops.append((list(qubits[c:]), [(q, True) for q in qubits[:c]], m))
+    return ops
+
+
 def matrix_embed(m: np.ndarray, gate_qubits, qubits) -> np.ndarray:
     m = np.ascontiguousarray(m, dtype=np.complex128)
     g = np.ascontiguousarray(gate_qubits, dtype=np.uint32)

@@ -254,8 +254,44 @@ int num_cus(int device) {
     return g_num_cus;
 }
 
+// Events around kernel launches on the handle's stream (QVD_FLAG_TIMING).
+struct LaunchTimer {
+    bool on = false;
+    hipStream_t stream = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> stage, other;
+    void begin(std::vector<std::pair<hipEvent_t, hipEvent_t>> &v) {
+        if (!on) return;
+        hipEvent_t a, b;
+        QVD_HIP(hipEventCreate(&a));
+        QVD_HIP(hipEventCreate(&b));
+        v.push_back({a, b});
+        QVD_HIP(hipEventRecord(a, stream));
+    }
+    void end(std::vector<std::pair<hipEvent_t, hipEvent_t>> &v) {
+        if (on) QVD_HIP(hipEventRecord(v.back().second, stream));
+    }
+    static double total(std::vector<std::pair<hipEvent_t, hipEvent_t>> &v) {
+        double t = 0;
+        for (auto &e : v) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) t += ms * 1e-3;
+        }
+        return t;
+    }
+    ~LaunchTimer() {
+        for (auto *v : {&stage, &other})
+            for (auto &e : *v) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+    }
+};
+
 template <typename T2>
-void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st) {
+void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, bool timing) {
+    LaunchTimer timer;
+    timer.on = timing;
+    timer.stream = sv->stream;
     // Lower every stage first so one upload serves the whole plan.
     std::vector<StageOp> sops;
     std::vector<T2> coefs;
@@ -283,17 +319,25 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st) 
             const unsigned per_cu = lds <= 80 * 1024 ? 2 : 1;
             const uint64_t want = (uint64_t)cus * per_cu * 2;
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, want);
+            timer.begin(timer.stage);
             launch_stage<T2>(s, d, dops.as<StageOp>(), dcoef.as<T2>(), grid, sv->stream);
             QVD_HIP(hipGetLastError());
+            timer.end(timer.stage);
             st.stages++;
             st.passes++;
         } else {
+            timer.begin(timer.other);
             apply_full<T2>(sv, s, step.op);
+            timer.end(timer.other);
             st.kernels++;
             st.passes++;
         }
     }
     sv->sync();
+    if (timing) {
+        st.stage_kernel_seconds = LaunchTimer::total(timer.stage);
+        st.other_kernel_seconds = LaunchTimer::total(timer.other);
+    }
 }
 
 uint64_t state_cap_bytes() {
@@ -488,7 +532,7 @@ int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_
         st.gates = num_ops;
         sv->bind();
         with_type(sv, [&](auto *s) {
-            run_plan(sv, s, steps, st);
+            run_plan(sv, s, steps, st, (opt.flags & QVD_FLAG_TIMING) != 0);
             return 0;
         });
         const double t2 = now();
@@ -669,6 +713,40 @@ int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, ui
         QVD_HIP(hipMemcpyAsync(out, dout.p, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
         sv->sync();
         sv->last_sample_seconds = now() - t0;
+    });
+}
+
+int qvd_sv_compare(const qvd_sv *a_c, const qvd_sv *b, double *max_abs, double *diff_sq, double *ref_sq) {
+    return guard([&] {
+        check_handle(a_c);
+        check_handle(b);
+        require(a_c->n == b->n, "states of different sizes");
+        require(a_c->device == b->device, "states on different devices");
+        qvd_sv *a = const_cast<qvd_sv *>(a_c);
+        a->bind();
+        QVD_HIP(hipStreamSynchronize(b->stream));
+        const unsigned blocks = 1024;
+        double *buf = a->scratch_doubles(3 * blocks);
+        with_type(a, [&](auto *x) {
+            with_type(b, [&](auto *y) {
+                launch_compare(x, y, a->len, buf, blocks, a->stream);
+                return 0;
+            });
+            return 0;
+        });
+        QVD_HIP(hipGetLastError());
+        std::vector<double> h(3 * blocks);
+        QVD_HIP(hipMemcpyAsync(h.data(), buf, h.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+        a->sync();
+        double m = 0, d = 0, r = 0;
+        for (unsigned i = 0; i < blocks; i++) {
+            m = std::max(m, h[3 * i]);
+            d += h[3 * i + 1];
+            r += h[3 * i + 2];
+        }
+        if (max_abs) *max_abs = std::sqrt(m);
+        if (diff_sq) *diff_sq = d;
+        if (ref_sq) *ref_sq = r;
     });
 }
 

@@ -39,6 +39,8 @@ void launch_stage(T2 *s, const StageDesc &d, const StageOp *ops, const T2 *coefs
 template <typename T2>
 void launch_masked_norm(const T2 *s, uint64_t len, uint64_t mask, uint64_t val, double *partial, unsigned blocks,
                         double *out, hipStream_t st);
+template <typename TA, typename TB>
+void launch_compare(const TA *a, const TB *b, uint64_t len, double *partial, unsigned blocks, hipStream_t st);
 uint32_t expect_segment_bits(uint32_t n);
 template <typename T2>
 void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, double *zz, hipStream_t st);

@@ -325,6 +325,40 @@ __global__ void k_sum_partials(const double *__restrict__ partial, uint32_t m, d
     }
 }
 
+// Per-block max |a-b|, sum |a-b|^2 and sum |b|^2 over two states.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(RED_THREADS) void k_compare(const TA *__restrict__ a, const TB *__restrict__ b,
+                                                         uint64_t len, double *__restrict__ partial) {
+    __shared__ double scratch[RED_THREADS / 64];
+    __shared__ double mx[RED_THREADS / 64];
+    double m = 0, d2 = 0, r2 = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        const TA x = a[i];
+        const TB y = b[i];
+        const double dr = (double)x.x - (double)y.x, di = (double)x.y - (double)y.y;
+        const double dd = dr * dr + di * di;
+        d2 += dd;
+        m = dd > m ? dd : m;
+        r2 += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(m, o, 64);
+        m = t > m ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
+    const double s1 = block_sum(d2, scratch);
+    const double s2 = block_sum(r2, scratch);
+    if (threadIdx.x == 0) {
+        double mm = 0;
+        for (int w = 0; w < RED_THREADS / 64; w++) mm = mx[w] > mm ? mx[w] : mm;
+        partial[3 * blockIdx.x] = mm;
+        partial[3 * blockIdx.x + 1] = s1;
+        partial[3 * blockIdx.x + 2] = s2;
+    }
+}
+
 // <Z_q> and <Z_q Z_q+1> in one pass. A block covers a segment of 2^B
 // contiguous amplitudes; per segment it writes 2B values: the total T, the
 // per-bit sums Q[q] = sum p [bit q] (q < B) and the per-pair sums
@@ -608,6 +642,16 @@ void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, doubl
     const uint32_t outs = 2 * n - 1;
     k_expect_combine<<<(outs + 63) / 64, 64, 0, st>>>(seg_sums, segs, B, n, z, zz);
 }
+
+template <typename TA, typename TB>
+void launch_compare(const TA *a, const TB *b, uint64_t len, double *partial, unsigned blocks, hipStream_t st) {
+    k_compare<TA, TB><<<blocks, RED_THREADS, 0, st>>>(a, b, len, partial);
+}
+template void launch_compare<float2, float2>(const float2 *, const float2 *, uint64_t, double *, unsigned, hipStream_t);
+template void launch_compare<float2, double2>(const float2 *, const double2 *, uint64_t, double *, unsigned, hipStream_t);
+template void launch_compare<double2, float2>(const double2 *, const float2 *, uint64_t, double *, unsigned, hipStream_t);
+template void launch_compare<double2, double2>(const double2 *, const double2 *, uint64_t, double *, unsigned,
+                                               hipStream_t);
 
 uint32_t expect_segment_bits(uint32_t n) { return n < EZ_TID_BITS + EZ_MAX_M_BITS ? n : EZ_TID_BITS + EZ_MAX_M_BITS; }
 

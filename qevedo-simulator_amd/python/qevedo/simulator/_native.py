@@ -18,13 +18,14 @@ LIB_PATH = os.environ.get("QVD_LIB", os.path.join(_PKG_ROOT, "lib", "libqvd_hip.
 QVD_OK, QVD_EINVAL, QVD_ENOMEM, QVD_EHIP, QVD_EUNSUPPORTED = 0, -1, -2, -3, -4
 C64, C128 = 0, 1
 PLANNER_HEURISTIC, PLANNER_LEARNED = 0, 1
+FLAG_TIMING = 1
 
 # Every symbol include/qvd_hip.h declares.
 EXPORTS = (
     "qvd_last_error", "qvd_device_info", "qvd_sv_create", "qvd_sv_destroy", "qvd_sv_num_qubits",
     "qvd_sv_precision", "qvd_sv_set_zero", "qvd_sv_set_amplitudes", "qvd_sv_get_amplitudes", "qvd_sv_apply",
     "qvd_sv_apply_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
-    "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_last_sample_seconds",
+    "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_compare", "qvd_sv_last_sample_seconds",
     "qvd_gate_matrix", "qvd_gate_lower",
 )
 
@@ -51,7 +52,7 @@ class Options(C.Structure):
         ("max_fused_qubits", C.c_uint32),
         ("tile_bits", C.c_int32),
         ("planner", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -64,6 +65,8 @@ class Stats(C.Structure):
         ("gate_seconds", C.c_double),
         ("measure_seconds", C.c_double),
         ("plan_seconds", C.c_double),
+        ("stage_kernel_seconds", C.c_double),
+        ("other_kernel_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -101,6 +104,7 @@ def lib():
         L.qvd_sv_measure.argtypes = [vp, u32, d, P(C.c_uint8)]
         L.qvd_sv_reset.argtypes = [vp, u32, d]
         L.qvd_sv_sample.argtypes = [vp, P(d), u64, P(u64)]
+        L.qvd_sv_compare.argtypes = [vp, vp, P(d), P(d), P(d)]
         L.qvd_sv_last_sample_seconds.argtypes = [vp, P(d)]
         L.qvd_gate_matrix.argtypes = [C.c_char_p, P(d), u32, P(d), u32, P(u32)]
         L.qvd_gate_lower.argtypes = [C.c_char_p, P(d), u32, P(u32), P(u32), P(d), u32]
@@ -233,10 +237,11 @@ class StateVector:
         cv = np.ascontiguousarray([1 if c[1] else 0 for c in controls], dtype=np.uint8)
         _check(lib().qvd_sv_apply(self.h, _p(m.view(np.float64), C.c_double), _p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size))
 
-    def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC) -> dict:
+    def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC,
+                  timing: bool = False) -> dict:
         """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default."""
         arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0)
+        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, FLAG_TIMING if timing else 0)
         st = Stats()
         _check(lib().qvd_sv_apply_ops(self.h, arr.arr, arr.n, C.byref(opt), C.byref(st)))
         return st.as_dict()
@@ -279,6 +284,12 @@ class StateVector:
         out = np.empty(u.size, dtype=np.uint64)
         _check(lib().qvd_sv_sample(self.h, _p(u, C.c_double), u.size, _p(out, C.c_uint64)))
         return out
+
+    def compare(self, reference: "StateVector") -> dict:
+        """max_abs and relative L2 distance to `reference`, computed on the GPU."""
+        m, d, r = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().qvd_sv_compare(self.h, reference.h, C.byref(m), C.byref(d), C.byref(r)))
+        return {"max_abs": m.value, "rel_l2": (d.value / r.value) ** 0.5 if r.value > 0 else float("nan")}
 
     def last_sample_seconds(self) -> float:
         v = C.c_double()

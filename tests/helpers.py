@@ -54,19 +54,7 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 def lower_with_oracle(gates) -> list:
     """gate_op (simulator.rs:106-119) over the oracle's gate table: controls
     stay separate, the matrix is the base gate's."""
-    ops = []
-    for name, qubits, params in gates:
-        full = oracle.gate_matrix(name, params)
-        c = oracle.gate_controls(name)
-        if c:
-            mask = (1 << c) - 1
-            dim = full.shape[0] >> c
-            idx = [(i << c) | mask for i in range(dim)]
-            base = full[np.ix_(idx, idx)]
-            ops.append((list(qubits[c:]), [(q, True) for q in qubits[:c]], base))
-        else:
-            ops.append((list(qubits), [], full))
-    return ops
+    return oracle.lower(gates)
 
 
 def full_operator(n: int, matrix: np.ndarray, qubits) -> np.ndarray:
