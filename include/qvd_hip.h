@@ -114,6 +114,12 @@ int qvd_sv_apply(qvd_sv *sv, const double *matrix, const uint32_t *targets, uint
 int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options *options,
                      qvd_stats *stats);
 
+/* Plan `ops` for a state of `num_qubits` qubits and `precision` exactly as
+ * qvd_sv_apply_ops would, without a device: fills gates, kernels, passes,
+ * stages and plan_seconds of `stats`. */
+int qvd_plan_ops(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops, const qvd_options *options,
+                 qvd_stats *stats);
+
 /* StateVector::norm_sqr (src/state.rs:172-177), accumulated in FP64. */
 int qvd_sv_norm_sqr(const qvd_sv *sv, double *out);
 /* StateVector::scale (src/state.rs:201-207). */

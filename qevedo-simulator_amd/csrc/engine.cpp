@@ -12,6 +12,8 @@
 #include "common.h"
 #include "kernels.h"
 #include "plan.h"
+#include "vprog.h"
+#include "vstage.h"
 
 using namespace qvd;
 
@@ -296,8 +298,25 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
     std::vector<StageOp> sops;
     std::vector<T2> coefs;
     std::vector<StageDesc> descs;
+    // Register-tile stages: programs, their coefficients (complex128 until
+    // upload), LDS ops and integer tables.
+    std::vector<VInstr> vins;
+    std::vector<C64> vco;
+    std::vector<StageOp> vlds;
+    std::vector<uint32_t> vints;
+    std::vector<VStageDesc> vdescs;
+    PendingDiag pend(sv->n);
+    const VGeom geom = vstage_geom<T2>();
     for (const Step &step : steps) {
         if (!step.is_stage) continue;
+        if (step.stage.reg_tile) {
+            VProgram p = build_vprogram(step.stage, sv->n, geom, pend, vco, vlds, vints, true);
+            require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
+            p.desc.instr_off = (uint32_t)vins.size();
+            vins.insert(vins.end(), p.instr.begin(), p.instr.end());
+            vdescs.push_back(p.desc);
+            continue;
+        }
         StageDesc d;
         lower_stage<T2>(step.stage, sv->n, sops, coefs, d);
         descs.push_back(d);
@@ -308,10 +327,34 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
         QVD_HIP(hipMemcpyAsync(dops.p, sops.data(), sops.size() * sizeof(StageOp), hipMemcpyHostToDevice, sv->stream));
         QVD_HIP(hipMemcpyAsync(dcoef.p, coefs.data(), coefs.size() * sizeof(T2), hipMemcpyHostToDevice, sv->stream));
     }
+    std::vector<T2> vco_t(vco.size());
+    for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
+    DBuf dvins(std::max<size_t>(vins.size(), 1) * sizeof(VInstr));
+    DBuf dvco(std::max<size_t>(vco_t.size(), 1) * sizeof(T2));
+    DBuf dvlds(std::max<size_t>(vlds.size(), 1) * sizeof(StageOp));
+    DBuf dvints(std::max<size_t>(vints.size(), 1) * sizeof(uint32_t));
+    if (!vins.empty())
+        QVD_HIP(hipMemcpyAsync(dvins.p, vins.data(), vins.size() * sizeof(VInstr), hipMemcpyHostToDevice, sv->stream));
+    if (!vco_t.empty())
+        QVD_HIP(hipMemcpyAsync(dvco.p, vco_t.data(), vco_t.size() * sizeof(T2), hipMemcpyHostToDevice, sv->stream));
+    if (!vlds.empty())
+        QVD_HIP(hipMemcpyAsync(dvlds.p, vlds.data(), vlds.size() * sizeof(StageOp), hipMemcpyHostToDevice, sv->stream));
+    if (!vints.empty())
+        QVD_HIP(hipMemcpyAsync(dvints.p, vints.data(), vints.size() * sizeof(uint32_t), hipMemcpyHostToDevice, sv->stream));
     const int cus = num_cus(sv->device);
-    size_t si = 0;
+    size_t si = 0, vi = 0;
     for (const Step &step : steps) {
-        if (step.is_stage) {
+        if (step.is_stage && step.stage.reg_tile) {
+            const VStageDesc &d = vdescs[vi++];
+            st.kernels += step.stage.ops.size();
+            const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, (uint64_t)cus);
+            timer.begin(timer.stage);
+            QVD_HIP(launch_vstage<T2>(s, d, dvins.as<VInstr>(), dvco.as<T2>(), dvlds.as<StageOp>(), dvints.as<uint32_t>(),
+                                      grid, sv->stream));
+            timer.end(timer.stage);
+            st.stages++;
+            st.passes++;
+        } else if (step.is_stage) {
             const StageDesc &d = descs[si++];
             st.kernels += d.num_ops;
             if (d.num_ops == 0) continue;
@@ -344,6 +387,85 @@ uint64_t state_cap_bytes() {
     const char *e = std::getenv("QVD_MAX_STATE_BYTES");
     if (!e || !*e) return 0;
     return std::strtoull(e, nullptr, 10);
+}
+
+// Validates the ops and plans them (heuristic planner) for a state of n
+// qubits in `precision`.
+std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64_t num_ops, const qvd_options &opt) {
+    require(num_ops == 0 || ops != nullptr, "null ops");
+    std::vector<Op> v;
+    v.reserve(num_ops);
+    for (uint64_t i = 0; i < num_ops; i++) {
+        const qvd_op &o = ops[i];
+        Op op = make_op(n, o.matrix, o.targets, o.num_targets, o.controls, o.control_values, o.num_controls);
+        if (op.targets.size() > MAX_GATE_QUBITS)
+            fail(QVD_EUNSUPPORTED, "ops on more than " + std::to_string(MAX_GATE_QUBITS) + " target qubits are not supported");
+        v.push_back(std::move(op));
+    }
+    if (opt.tile_bits < 0) return plan_unblocked(v, opt.max_fused_qubits);
+    TileConfig tc = default_tiles(precision);
+    const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    PlanParams p;
+    p.num_qubits = n;
+    p.max_fused = std::max<uint32_t>(opt.max_fused_qubits, 1);
+    p.max_stage_op = 4;
+    if (opt.tile_bits == 0 && n >= vg.L()) {
+        // Register-tile stages: lane bits are qubits 0..5.
+        p.reg_tile = true;
+        p.tile_bits = vg.L();
+        p.contig = 6;
+    } else {
+        p.tile_bits = opt.tile_bits > 0 ? (uint32_t)opt.tile_bits : tc.L;
+        require(p.tile_bits <= 14 - (precision == QVD_C128 ? 1 : 0) && p.tile_bits >= 1,
+                "tile_bits out of range (1..14 for complex64, 1..13 for complex128)");
+        p.contig = std::min(tc.contig, p.tile_bits);
+    }
+    std::vector<Step> steps = plan_heuristic(v, p);
+    if (!p.reg_tile) return steps;
+    // Each register-tile stage copies its coefficients to LDS: split stages
+    // whose coefficients would exceed the budget (each part is a pass).
+    const size_t budget = VCONST_BYTES / (precision == QVD_C64 ? sizeof(float2) : sizeof(double2));
+    const size_t scale = (64u << vg.w_bits) + (1u << vg.r_bits) + 2 * n;
+    auto entries = [](const Op &op) -> size_t {
+        const size_t k = op.targets.size();
+        if (op.diagonal) return size_t(1) << k;
+        return size_t(1) << (2 * k);
+    };
+    std::vector<Step> out;
+    for (Step &s : steps) {
+        if (!s.is_stage) {
+            out.push_back(std::move(s));
+            continue;
+        }
+        Step cur;
+        cur.stage.local = s.stage.local;
+        cur.stage.reg_tile = true;
+        size_t used = scale;
+        for (Op &op : s.stage.ops) {
+            const size_t e = entries(op);
+            if (used + e > budget && !cur.stage.ops.empty()) {
+                out.push_back(cur);
+                cur.stage.ops.clear();
+                used = scale;
+            }
+            used += e;
+            cur.stage.ops.push_back(std::move(op));
+        }
+        if (!cur.stage.ops.empty()) out.push_back(std::move(cur));
+    }
+    return out;
+}
+
+void count_plan(const std::vector<Step> &steps, qvd_stats &st) {
+    for (const Step &s : steps) {
+        st.passes++;
+        if (s.is_stage) {
+            st.stages++;
+            st.kernels += s.stage.ops.size();
+        } else {
+            st.kernels++;
+        }
+    }
 }
 
 }  // namespace
@@ -496,37 +618,31 @@ int qvd_sv_apply(qvd_sv *sv, const double *matrix, const uint32_t *targets, uint
     });
 }
 
+
+int qvd_plan_ops(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops,
+                 const qvd_options *options, qvd_stats *stats) {
+    return guard([&] {
+        require(precision == QVD_C64 || precision == QVD_C128, "precision must be QVD_C64 or QVD_C128");
+        require(num_qubits <= 34, "a state vector of more than 34 qubits does not fit in memory");
+        qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
+        if (options) opt = *options;
+        const double t0 = now();
+        std::vector<Step> steps = make_plan(num_qubits, precision, ops, num_ops, opt);
+        qvd_stats st{};
+        st.gates = num_ops;
+        st.plan_seconds = now() - t0;
+        count_plan(steps, st);
+        if (stats) *stats = st;
+    });
+}
+
 int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options *options, qvd_stats *stats) {
     return guard([&] {
         check_handle(sv);
-        require(num_ops == 0 || ops != nullptr, "null ops");
         const double t0 = now();
         qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
         if (options) opt = *options;
-        std::vector<Op> v;
-        v.reserve(num_ops);
-        for (uint64_t i = 0; i < num_ops; i++) {
-            const qvd_op &o = ops[i];
-            Op op = make_op(sv->n, o.matrix, o.targets, o.num_targets, o.controls, o.control_values, o.num_controls);
-            if (op.targets.size() > MAX_GATE_QUBITS)
-                fail(QVD_EUNSUPPORTED, "ops on more than " + std::to_string(MAX_GATE_QUBITS) + " target qubits are not supported");
-            v.push_back(std::move(op));
-        }
-        std::vector<Step> steps;
-        if (opt.tile_bits < 0) {
-            steps = plan_unblocked(v, opt.max_fused_qubits);
-        } else {
-            TileConfig tc = default_tiles(sv->precision);
-            PlanParams p;
-            p.num_qubits = sv->n;
-            p.tile_bits = opt.tile_bits > 0 ? (uint32_t)opt.tile_bits : tc.L;
-            require(p.tile_bits <= 14 - (sv->precision == QVD_C128 ? 1 : 0) && p.tile_bits >= 1,
-                    "tile_bits out of range (1..14 for complex64, 1..13 for complex128)");
-            p.contig = std::min(tc.contig, p.tile_bits);
-            p.max_fused = std::max<uint32_t>(opt.max_fused_qubits, 1);
-            p.max_stage_op = 4;
-            steps = plan_heuristic(v, p);
-        }
+        std::vector<Step> steps = make_plan(sv->n, sv->precision, ops, num_ops, opt);
         const double t1 = now();
         qvd_stats st{};
         st.gates = num_ops;

@@ -159,16 +159,23 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
                 stage_ops.push_back(ops[i]);
             }
             remaining -= taken.size();
-            // Fuse only ops on tile qubits: merging a non-tile control or
-            // diagonal target into a dense matrix would need it in the tile.
-            std::vector<Op> fused = fuse_where(stage_ops, std::min<size_t>(max_fused, p.max_stage_op), [&](const Op &op) {
-                for (uint32_t q : op.qubits())
-                    if (!is_local[q]) return false;
-                return true;
-            });
             Step s;
             s.stage.local = local;
-            s.stage.ops = std::move(fused);
+            s.stage.reg_tile = p.reg_tile;
+            if (p.reg_tile) {
+                // The register-tile program merges 1-qubit runs itself and
+                // keeps gates in unit form; wider fusion would only add
+                // arithmetic to a compute-bound tile.
+                s.stage.ops = std::move(stage_ops);
+            } else {
+                // Fuse only ops on tile qubits: merging a non-tile control or
+                // diagonal target into a dense matrix would need it in the tile.
+                s.stage.ops = fuse_where(stage_ops, std::min<size_t>(max_fused, p.max_stage_op), [&](const Op &op) {
+                    for (uint32_t q : op.qubits())
+                        if (!is_local[q]) return false;
+                    return true;
+                });
+            }
             steps.push_back(std::move(s));
         }
         if (remaining == 0) break;

@@ -27,7 +27,8 @@ std::vector<Op> fuse(const std::vector<Op> &ops, size_t max_qubits);
 // on non-tile qubits, resolved per tile).
 struct Stage {
     std::vector<uint32_t> local;  // tile bit j -> physical qubit local[j], ascending
-    std::vector<Op> ops;          // fused ops, program order
+    std::vector<Op> ops;          // program order (fused for LDS-tile stages)
+    bool reg_tile = false;        // run by the register-tile kernel (ops unfused)
 };
 
 // One step of a plan: an LDS-tile stage, or a single op applied as its own
@@ -44,6 +45,7 @@ struct PlanParams {
     uint32_t contig = 5;      // lowest qubits always in the tile
     uint32_t max_fused = 4;
     uint32_t max_stage_op = 4;  // widest dense op applied inside a tile
+    bool reg_tile = false;      // stages for the register-tile kernel: ops kept unfused
 };
 
 // The heuristic plan: src/blocking.rs's extract / plan_swaps loop with the

@@ -1,0 +1,470 @@
+// Register-tile stage programs (see vprog.h).
+//
+// The program generator orders a stage's ops by their true dependencies
+// (the dependency-aware extraction of src/blocking.rs:157-179, applied to
+// the register bits of a tile instead of the local qubits of a region),
+// inserts LDS exchanges when a gate needs a wave bit, merges runs of
+// 1-qubit gates on one qubit, and keeps 1-qubit gates in unit form with
+// their diagonal factors carried forward.
+#include "vprog.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+
+namespace qvd {
+
+namespace {
+
+enum Role : uint8_t { ROLE_LANE, ROLE_WAVE, ROLE_REG, ROLE_NONE };
+
+struct Roles {
+    std::vector<uint8_t> wmap, rmap;  // wave / register slot -> tile bit
+    std::vector<Role> role;           // tile bit -> role
+    std::vector<uint8_t> slot;        // tile bit -> slot index
+
+    void assign(uint32_t L, const std::vector<uint8_t> &w, const std::vector<uint8_t> &r) {
+        wmap = w;
+        rmap = r;
+        role.assign(L, ROLE_NONE);
+        slot.assign(L, 0);
+        for (uint32_t b = 0; b < 6; b++) {
+            role[b] = ROLE_LANE;
+            slot[b] = (uint8_t)b;
+        }
+        for (size_t s = 0; s < w.size(); s++) {
+            role[w[s]] = ROLE_WAVE;
+            slot[w[s]] = (uint8_t)s;
+        }
+        for (size_t s = 0; s < r.size(); s++) {
+            role[r[s]] = ROLE_REG;
+            slot[r[s]] = (uint8_t)s;
+        }
+    }
+};
+
+using M2 = std::array<C64, 4>;  // row-major 2x2
+
+M2 mul2(const M2 &a, const M2 &b) {
+    return {cmul(a[0], b[0]) + cmul(a[1], b[2]), cmul(a[0], b[1]) + cmul(a[1], b[3]),
+            cmul(a[2], b[0]) + cmul(a[3], b[2]), cmul(a[2], b[1]) + cmul(a[3], b[3])};
+}
+
+struct Gen {
+    const Stage &st;
+    uint32_t n;
+    VGeom g;
+    PendingDiag &pend;
+    std::vector<C64> &coefs;
+    std::vector<StageOp> &ldsops;
+    std::vector<uint32_t> &ints;
+    std::vector<int> tbit;  // physical -> tile bit (-1 if not in the tile)
+    Roles roles;
+    std::vector<VInstr> out;
+    // Pending non-diagonal 1-qubit product per physical qubit (applied
+    // after the pending diagonal).
+    std::vector<M2> pmat;
+    std::vector<char> has_pmat;
+
+    Gen(const Stage &s, uint32_t n_, const VGeom &g_, PendingDiag &p, std::vector<C64> &c, std::vector<StageOp> &l,
+        std::vector<uint32_t> &ii)
+        : st(s), n(n_), g(g_), pend(p), coefs(c), ldsops(l), ints(ii), tbit(n_, -1), pmat(n_), has_pmat(n_, 0) {
+        for (size_t j = 0; j < s.local.size(); j++) tbit[s.local[j]] = (int)j;
+    }
+
+    // Where a physical qubit's bit lives now, as a DIAG target encoding.
+    uint32_t encode(uint32_t q) const {
+        const int b = tbit[q];
+        if (b < 0) return VT_GLOBAL | q;
+        switch (roles.role[b]) {
+            case ROLE_LANE: return VT_THREAD | (uint32_t)b;
+            case ROLE_WAVE: return VT_THREAD | (6u + roles.slot[b]);
+            default: return VT_REG | roles.slot[b];
+        }
+    }
+
+    void add_controls(VInstr &v, const std::vector<std::pair<uint32_t, bool>> &controls) const {
+        for (auto &c : controls) {
+            const uint32_t e = encode(c.first);
+            const uint32_t x = e & 0xffffu;
+            switch (e & VT_KIND) {
+                case VT_THREAD:
+                    v.tmask |= 1u << x;
+                    if (c.second) v.tval |= 1u << x;
+                    break;
+                case VT_REG:
+                    v.rmask |= 1u << x;
+                    if (c.second) v.rval |= 1u << x;
+                    break;
+                default:
+                    v.gmask |= uint64_t(1) << x;
+                    if (c.second) v.gval |= uint64_t(1) << x;
+            }
+        }
+    }
+
+    VInstr blank(uint32_t op) const {
+        VInstr v{};
+        v.op = op;
+        return v;
+    }
+
+    // Emits a 1-qubit gate m (row-major, in the current frame) on q.
+    void emit_u1(uint32_t q, const M2 &m, uint32_t form, const std::vector<std::pair<uint32_t, bool>> &controls) {
+        const int b = tbit[q];
+        require(b >= 0 && roles.role[b] != ROLE_WAVE, "internal: 1-qubit gate on a wave bit");
+        VInstr v = blank(roles.role[b] == ROLE_LANE ? V_U1L : V_U1R);
+        v.slot = roles.role[b] == ROLE_LANE ? (uint32_t)b : roles.slot[b];
+        v.form = form;
+        add_controls(v, controls);
+        v.coef = (uint32_t)coefs.size();
+        for (const C64 &c : m) coefs.push_back(c);
+        out.push_back(v);
+    }
+
+    // Applies the pending product on q: U = P * D_q = D_new * M with M in
+    // unit form; emits M and keeps D_new pending.
+    void flush_pmat(uint32_t q) {
+        if (!has_pmat[q]) return;
+        has_pmat[q] = 0;
+        const M2 &p = pmat[q];
+        const M2 u = {cmul(p[0], pend.d0[q]), cmul(p[1], pend.d1[q]), cmul(p[2], pend.d0[q]), cmul(p[3], pend.d1[q])};
+        if (nsq(u[1]) == 0.0 && nsq(u[2]) == 0.0) {  // diagonal after all
+            pend.d0[q] = u[0];
+            pend.d1[q] = u[3];
+            return;
+        }
+        if (nsq(u[0]) >= nsq(u[1])) {
+            // [[u0, u1], [u2, u3]] = diag(u0, u3) [[1, u1/u0], [u2/u3, 1]]
+            emit_u1(q, {C64(1, 0), u[1] / u[0], u[2] / u[3], C64(1, 0)}, FORM_UNIT_DIAG, {});
+            pend.d0[q] = u[0];
+            pend.d1[q] = u[3];
+        } else {
+            // [[u0, u1], [u2, u3]] = diag(u1, u2) [[u0/u1, 1], [1, u3/u2]]
+            emit_u1(q, {u[0] / u[1], C64(1, 0), C64(1, 0), u[3] / u[2]}, FORM_UNIT_ANTI, {});
+            pend.d0[q] = u[1];
+            pend.d1[q] = u[2];
+        }
+    }
+
+    // A non-diagonal op on targets T (with controls) in the frame where the
+    // pending diagonals D_T stay pending: D^-1 U D.
+    HMatrix conjugate(const Op &op) const {
+        HMatrix m = op.m;
+        const size_t d = m.dim();
+        for (size_t r = 0; r < d; r++)
+            for (size_t c = 0; c < d; c++) {
+                C64 fr(1, 0), fc(1, 0);
+                for (size_t j = 0; j < op.targets.size(); j++) {
+                    const uint32_t q = op.targets[j];
+                    fr = cmul(fr, ((r >> j) & 1) ? pend.d1[q] : pend.d0[q]);
+                    fc = cmul(fc, ((c >> j) & 1) ? pend.d1[q] : pend.d0[q]);
+                }
+                m.at(r, c) = cmul(m.at(r, c), fc) / fr;
+            }
+        return m;
+    }
+
+    void emit_diag(const Op &op) {
+        for (uint32_t q : op.qubits()) flush_pmat(q);
+        VInstr v = blank(V_DIAG);
+        v.k = (uint32_t)op.targets.size();
+        for (uint32_t j = 0; j < v.k; j++) v.tgt[j] = encode(op.targets[j]);
+        add_controls(v, op.controls);
+        v.coef = (uint32_t)coefs.size();
+        bool sign = true;
+        uint64_t neg = 0;
+        for (size_t i = 0; i < op.m.dim(); i++) {
+            const C64 e = op.m.at(i, i);
+            coefs.push_back(e);
+            if (e == C64(-1, 0)) neg |= uint64_t(1) << i;
+            else if (e != C64(1, 0)) sign = false;
+        }
+        if (sign) {  // entries +-1: a sign flip where the mask bit is set
+            v.form = 1;
+            v.neg = neg;
+        }
+        out.push_back(v);
+    }
+
+    void emit_ldsop(const Op &op) {
+        for (uint32_t q : op.qubits()) flush_pmat(q);
+        HMatrix m = conjugate(op);
+        StageOp so{};
+        so.kind = OP_DENSE;
+        so.k = (uint32_t)op.targets.size();
+        std::vector<uint32_t> ins;
+        for (uint32_t j = 0; j < so.k; j++) {
+            const int b = tbit[op.targets[j]];
+            require(b >= 0, "internal: LDS op on a non-tile target");
+            so.tgt[j] = (uint32_t)b;
+            ins.push_back((uint32_t)b);
+        }
+        for (auto &c : op.controls) {
+            const int b = tbit[c.first];
+            if (b >= 0) {
+                so.lc_mask |= 1u << b;
+                if (c.second) so.lc_val |= 1u << b;
+                ins.push_back((uint32_t)b);
+            } else {
+                so.gc_mask |= uint64_t(1) << c.first;
+                if (c.second) so.gc_val |= uint64_t(1) << c.first;
+            }
+        }
+        std::sort(ins.begin(), ins.end());
+        so.nins = (uint32_t)ins.size();
+        for (size_t j = 0; j < ins.size(); j++) so.ins[j] = ins[j];
+        so.coef_off = (uint32_t)coefs.size();
+        for (const C64 &c : m.a) coefs.push_back(c);
+        VInstr v = blank(V_LDSOP);
+        v.aux = (uint32_t)ldsops.size();
+        v.gmask = so.gc_mask;
+        v.gval = so.gc_val;
+        ldsops.push_back(so);
+        out.push_back(v);
+    }
+
+    // Applies every pending diagonal: one SCALE with per-thread,
+    // per-register and per-tile (non-tile qubit) factors.
+    void emit_scale() {
+        const uint32_t W = 64u << g.w_bits, R = 1u << g.r_bits;
+        std::vector<C64> ft(W, C64(1, 0)), fr(R, C64(1, 0));
+        std::vector<uint32_t> globals;
+        bool any = false;
+        for (uint32_t q = 0; q < n; q++) {
+            if (pend.identity(q)) continue;
+            any = true;
+            const uint32_t e = encode(q), x = e & 0xffffu;
+            switch (e & VT_KIND) {
+                case VT_THREAD:
+                    for (uint32_t t = 0; t < W; t++) ft[t] = cmul(ft[t], ((t >> x) & 1) ? pend.d1[q] : pend.d0[q]);
+                    break;
+                case VT_REG:
+                    for (uint32_t j = 0; j < R; j++) fr[j] = cmul(fr[j], ((j >> x) & 1) ? pend.d1[q] : pend.d0[q]);
+                    break;
+                default: globals.push_back(q);
+            }
+        }
+        if (!any) return;
+        VInstr v = blank(V_SCALE);
+        v.coef = (uint32_t)coefs.size();
+        coefs.insert(coefs.end(), ft.begin(), ft.end());
+        coefs.insert(coefs.end(), fr.begin(), fr.end());
+        v.k = (uint32_t)globals.size();
+        for (uint32_t q : globals) {
+            coefs.push_back(pend.d0[q]);
+            coefs.push_back(pend.d1[q]);
+        }
+        v.aux = (uint32_t)ints.size();
+        ints.insert(ints.end(), globals.begin(), globals.end());
+        // Uniform factors: a plain complex scale.
+        bool uniform = globals.empty();
+        for (uint32_t t = 1; t < W && uniform; t++) uniform = ft[t] == ft[0];
+        for (uint32_t j = 1; j < R && uniform; j++) uniform = fr[j] == fr[0];
+        v.form = uniform ? 1 : 0;
+        if (uniform) coefs[v.coef] = cmul(ft[0], fr[0]);
+        out.push_back(v);
+        for (uint32_t q = 0; q < n; q++) {
+            pend.d0[q] = C64(1, 0);
+            pend.d1[q] = C64(1, 0);
+        }
+    }
+
+    // Tile bits an op needs outside the wave bits.
+    void needs(const Op &op, std::vector<uint32_t> &outb) const {
+        outb.clear();
+        if (op.diagonal) return;
+        if (op.targets.size() != 1) return;  // multi-qubit dense: applied in LDS
+        const int b = tbit[op.targets[0]];
+        if (b >= 6) outb.push_back((uint32_t)b);
+    }
+
+    bool runnable(const Op &op) const {
+        std::vector<uint32_t> nb;
+        needs(op, nb);
+        for (uint32_t b : nb)
+            if (roles.role[b] == ROLE_WAVE) return false;
+        return true;
+    }
+
+    template <typename F>
+    std::vector<size_t> extract(const std::vector<char> &done, F &&can_run) const {
+        std::vector<size_t> taken;
+        std::vector<uint32_t> blocked;
+        for (size_t i = 0; i < st.ops.size(); i++) {
+            if (done[i]) continue;
+            const std::vector<uint32_t> oq = st.ops[i].qubits();
+            bool touches = false;
+            for (uint32_t q : oq) touches |= contains(blocked, q);
+            if (!touches && can_run(st.ops[i])) {
+                taken.push_back(i);
+            } else {
+                for (uint32_t q : oq)
+                    if (!contains(blocked, q)) blocked.push_back(q);
+            }
+        }
+        return taken;
+    }
+
+    // The register bits the coming ops want, in order of need (capacity r).
+    std::vector<uint32_t> lookahead(const std::vector<char> &done) const {
+        std::vector<uint32_t> wanted, nb;
+        extract(done, [&](const Op &op) {
+            needs(op, nb);
+            std::vector<uint32_t> fresh;
+            for (uint32_t b : nb)
+                if (!contains(wanted, b)) fresh.push_back(b);
+            if (wanted.size() + fresh.size() > g.r_bits) return false;
+            wanted.insert(wanted.end(), fresh.begin(), fresh.end());
+            return true;
+        });
+        return wanted;
+    }
+
+    // New role map: wanted bits (and then current register bits) as
+    // register bits, the rest as wave bits.
+    void choose_roles(const std::vector<uint32_t> &wanted, bool initial, std::vector<uint8_t> &w,
+                      std::vector<uint8_t> &r) const {
+        const uint32_t L = (uint32_t)st.local.size();
+        r.clear();
+        w.clear();
+        for (uint32_t b : wanted)
+            if (r.size() < g.r_bits) r.push_back((uint8_t)b);
+        if (!initial)
+            for (uint8_t b : roles.rmap)
+                if (r.size() < g.r_bits && !contains(std::vector<uint32_t>(r.begin(), r.end()), b)) r.push_back(b);
+        for (uint32_t b = 6; b < L && r.size() < g.r_bits; b++)
+            if (!contains(std::vector<uint32_t>(r.begin(), r.end()), b)) r.push_back((uint8_t)b);
+        for (uint32_t b = 6; b < L; b++)
+            if (!contains(std::vector<uint32_t>(r.begin(), r.end()), b)) w.push_back((uint8_t)b);
+        // Keep register slots of bits that stay registers, so only moved bits
+        // change places (fewer instructions depend on the slot order).
+        if (!initial) {
+            std::vector<uint8_t> keep(g.r_bits, 0xff);
+            std::vector<uint8_t> rest;
+            for (uint8_t b : r) {
+                auto it = std::find(roles.rmap.begin(), roles.rmap.end(), b);
+                if (it != roles.rmap.end()) keep[it - roles.rmap.begin()] = b;
+                else rest.push_back(b);
+            }
+            size_t k = 0;
+            for (auto &s : keep)
+                if (s == 0xff) s = rest[k++];
+            r = keep;
+            std::vector<uint8_t> wkeep(g.w_bits, 0xff), wrest;
+            for (uint8_t b : w) {
+                auto it = std::find(roles.wmap.begin(), roles.wmap.end(), b);
+                if (it != roles.wmap.end()) wkeep[it - roles.wmap.begin()] = b;
+                else wrest.push_back(b);
+            }
+            k = 0;
+            for (auto &s : wkeep)
+                if (s == 0xff) s = wrest[k++];
+            w = wkeep;
+        }
+    }
+
+    void emit_op(const Op &op) {
+        if (op.diagonal) {
+            // 1-qubit diagonal: folded into the pending diagonal (or the
+            // pending product when one is open).
+            if (op.controls.empty() && op.targets.size() == 1) {
+                const uint32_t q = op.targets[0];
+                const C64 a = op.m.at(0, 0), b = op.m.at(1, 1);
+                if (has_pmat[q]) {
+                    M2 &p = pmat[q];
+                    p = {cmul(a, p[0]), cmul(a, p[1]), cmul(b, p[2]), cmul(b, p[3])};
+                } else {
+                    pend.d0[q] = cmul(a, pend.d0[q]);
+                    pend.d1[q] = cmul(b, pend.d1[q]);
+                }
+                return;
+            }
+            emit_diag(op);
+            return;
+        }
+        if (op.targets.size() == 1) {
+            const uint32_t q = op.targets[0];
+            for (auto &c : op.controls) flush_pmat(c.first);
+            if (op.controls.empty()) {
+                const M2 u = {op.m.at(0, 0), op.m.at(0, 1), op.m.at(1, 0), op.m.at(1, 1)};
+                if (has_pmat[q]) pmat[q] = mul2(u, pmat[q]);
+                else {
+                    pmat[q] = u;
+                    has_pmat[q] = 1;
+                }
+                return;
+            }
+            flush_pmat(q);
+            HMatrix m = conjugate(op);
+            emit_u1(q, {m.at(0, 0), m.at(0, 1), m.at(1, 0), m.at(1, 1)}, FORM_FULL, op.controls);
+            return;
+        }
+        emit_ldsop(op);
+    }
+
+    VProgram run(bool flush_all) {
+        const uint32_t L = (uint32_t)st.local.size();
+        require(L == g.L(), "internal: stage tile size does not match the register tile");
+        for (uint32_t b = 0; b < 6; b++) require(st.local[b] == b, "internal: lane bits must be qubits 0..5");
+        std::vector<char> done(st.ops.size(), 0);
+        std::vector<uint8_t> w, r;
+        choose_roles(lookahead(done), true, w, r);
+        roles.assign(L, w, r);
+        VProgram p;
+        p.desc = VStageDesc{};
+        for (uint32_t s = 0; s < g.w_bits; s++) p.desc.wmap0[s] = roles.wmap[s];
+        for (uint32_t s = 0; s < g.r_bits; s++) p.desc.rmap0[s] = roles.rmap[s];
+        size_t remaining = st.ops.size();
+        while (remaining) {
+            std::vector<size_t> taken = extract(done, [&](const Op &op) { return runnable(op); });
+            for (size_t i : taken) {
+                done[i] = 1;
+                emit_op(st.ops[i]);
+            }
+            remaining -= taken.size();
+            if (!remaining) break;
+            // Exchange: the wanted bits become register bits. Pending
+            // products on bits that change role are flushed first, since
+            // their slot is about to move.
+            std::vector<uint32_t> wanted = lookahead(done);
+            choose_roles(wanted, false, w, r);
+            for (uint32_t q = 0; q < n; q++) flush_pmat(q);
+            VInstr v = blank(V_XCHG);
+            for (uint32_t s = 0; s < g.w_bits; s++) v.wmap[s] = w[s];
+            for (uint32_t s = 0; s < g.r_bits; s++) v.rmap[s] = r[s];
+            require(w != roles.wmap || r != roles.rmap, "internal: exchange without progress");
+            roles.assign(L, w, r);
+            out.push_back(v);
+        }
+        for (uint32_t q = 0; q < n; q++) flush_pmat(q);
+        if (flush_all) emit_scale();
+        for (uint32_t s = 0; s < g.w_bits; s++) p.desc.wmapF[s] = roles.wmap[s];
+        for (uint32_t s = 0; s < g.r_bits; s++) p.desc.rmapF[s] = roles.rmap[s];
+        p.instr = std::move(out);
+        return p;
+    }
+};
+
+}  // namespace
+
+VProgram build_vprogram(const Stage &stage, uint32_t n, const VGeom &g, PendingDiag &pending,
+                        std::vector<C64> &coefs, std::vector<StageOp> &ldsops, std::vector<uint32_t> &ints,
+                        bool flush_all) {
+    Gen gen(stage, n, g, pending, coefs, ldsops, ints);
+    const uint32_t cbegin = (uint32_t)coefs.size();
+    VProgram p = gen.run(flush_all);
+    VStageDesc &d = p.desc;
+    d.cbegin = cbegin;
+    d.ccount = (uint32_t)coefs.size() - cbegin;
+    d.L = (uint32_t)stage.local.size();
+    for (uint32_t j = 0; j < d.L; j++) d.local_pos[j] = stage.local[j];
+    d.nglob = 0;
+    for (uint32_t q = 0; q < n; q++)
+        if (gen.tbit[q] < 0) d.glob_pos[d.nglob++] = q;
+    d.num_tiles = uint64_t(1) << d.nglob;
+    d.num_instr = (uint32_t)p.instr.size();
+    return p;
+}
+
+}  // namespace qvd

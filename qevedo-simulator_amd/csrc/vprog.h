@@ -1,0 +1,113 @@
+// Register-tile stage programs.
+//
+// A register-tile stage keeps its tile of 2^L amplitudes in VGPRs: thread t
+// of a workgroup of W = 64 * 2^w threads holds R = 2^r amplitudes, and
+// L = 6 + w + r. Tile bits play three roles:
+//   - lane bits: tile bits 0..5 (physical qubits 0..5, so every wave-wide
+//     access is 64 consecutive amplitudes); gates on them use cross-lane
+//     shuffles;
+//   - wave bits: the remaining bits of the thread index; a gate on one needs
+//     an exchange through LDS that makes it a register bit;
+//   - register bits: the index into the thread's registers; gates on them
+//     are thread-local arithmetic.
+// A program is a list of VInstr the kernel interprets in order; every
+// workgroup runs the same program on each of its tiles.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "device.h"
+#include "plan.h"
+
+namespace qvd {
+
+enum VOp : uint32_t {
+    V_U1R = 0,   // 1-qubit gate on a register bit
+    V_U1L = 1,   // 1-qubit gate on a lane bit
+    V_DIAG = 2,  // diagonal on up to 6 bits (thread, register or non-tile)
+    V_XCHG = 3,  // exchange wave bits and register bits through LDS
+    V_LDSOP = 4, // any op of up to 4 targets, applied with the tile in LDS
+    V_SCALE = 5, // multiply by per-thread x per-register x per-tile factors
+};
+
+// Forms of a 1-qubit gate's coefficients c[0..3]:
+//   FORM_FULL      [[c0, c1], [c2, c3]]
+//   FORM_UNIT_DIAG [[1, c1], [c2, 1]]   (4 fused multiply-adds per amplitude)
+//   FORM_UNIT_ANTI [[c0, 1], [1, c3]]
+enum VForm : uint32_t { FORM_FULL = 0, FORM_UNIT_DIAG = 1, FORM_UNIT_ANTI = 2 };
+
+constexpr uint32_t VT_THREAD = 0u << 16;  // target bit is bit x of the thread index
+constexpr uint32_t VT_REG = 1u << 16;     // bit x of the register index
+constexpr uint32_t VT_GLOBAL = 2u << 16;  // physical qubit x outside the tile
+constexpr uint32_t VT_KIND = 3u << 16;
+
+constexpr uint32_t VMAX_W_BITS = 4;
+constexpr uint32_t VMAX_R_BITS = 6;
+
+struct VInstr {
+    uint32_t op;
+    uint32_t form;
+    uint32_t slot;  // U1R: register bit; U1L: lane bit
+    uint32_t k;     // DIAG: number of targets; SCALE: number of non-tile factors
+    uint32_t tmask, tval;  // controls on thread-index bits
+    uint32_t rmask, rval;  // controls on register-index bits
+    uint64_t gmask, gval;  // controls on non-tile qubits (checked per tile)
+    uint64_t neg;          // DIAG, form 1: entries equal to -1 (the others are +1)
+    uint32_t coef;         // first coefficient (complex) of the op
+    uint32_t aux;          // LDSOP: index of its StageOp; SCALE: first non-tile qubit in the int array
+    uint32_t tgt[MAX_GATE_QUBITS];            // DIAG targets (VT_* | bit)
+    uint8_t wmap[VMAX_W_BITS];                // XCHG: tile bit of each wave bit
+    uint8_t rmap[VMAX_R_BITS];                // XCHG: tile bit of each register bit
+    uint8_t pad[6];
+};
+
+// One register-tile stage as launched.
+struct VStageDesc {
+    uint32_t L;
+    uint32_t nglob;
+    uint32_t num_instr;
+    uint32_t instr_off;
+    uint64_t num_tiles;
+    uint32_t local_pos[MAX_QUBITS];  // tile bit -> physical qubit (ascending)
+    uint32_t glob_pos[MAX_QUBITS];   // tile-index bit -> physical qubit
+    uint8_t wmap0[VMAX_W_BITS];      // role map at load
+    uint8_t rmap0[VMAX_R_BITS];
+    uint8_t wmapF[VMAX_W_BITS];      // role map at store
+    uint8_t rmapF[VMAX_R_BITS];
+    uint32_t cbegin, ccount;         // the stage's coefficients, copied to LDS once per workgroup
+};
+
+// LDS budget of a stage's coefficient block, in bytes.
+constexpr uint32_t VCONST_BYTES = 24 * 1024;
+
+// Geometry of the register-tile kernel for one precision.
+struct VGeom {
+    uint32_t w_bits;  // wave bits (W = 64 << w_bits threads)
+    uint32_t r_bits;  // register bits
+    uint32_t L() const { return 6 + w_bits + r_bits; }
+};
+
+// Per-qubit diagonal factors not yet applied to the amplitudes: the engine
+// keeps 1-qubit gates in unit form (D * M with M's diagonal or off-diagonal
+// equal to 1) and carries D forward to the next gate on that qubit, where it
+// is absorbed, or to the end of the stage, where a SCALE applies all of them.
+struct PendingDiag {
+    std::vector<C64> d0, d1;  // per physical qubit
+    explicit PendingDiag(uint32_t n = 0) : d0(n, C64(1, 0)), d1(n, C64(1, 0)) {}
+    bool identity(uint32_t q) const { return d0[q] == C64(1, 0) && d1[q] == C64(1, 0); }
+};
+
+// Builds the program of one stage. `coefs` (complex128) and `ldsops` are
+// appended to; the caller converts coefficients to the state's precision.
+struct VProgram {
+    std::vector<VInstr> instr;
+    VStageDesc desc;
+};
+
+VProgram build_vprogram(const Stage &stage, uint32_t n, const VGeom &g, PendingDiag &pending,
+                        std::vector<C64> &coefs, std::vector<StageOp> &ldsops, std::vector<uint32_t> &ints,
+                        bool flush_all);
+
+}  // namespace qvd

@@ -1,0 +1,432 @@
+// Register-tile stage kernel (gfx950).
+//
+// One workgroup of W = 64 << WB threads owns a tile of 2^L amplitudes,
+// L = 6 + WB + RB, held in VGPRs: thread t keeps R = 2^RB amplitudes x[0..R).
+// Lane bits are tile bits 0..5 (physical qubits 0..5: each wave-wide load or
+// store is 64 consecutive amplitudes); the wave and register bits hold the
+// other tile bits in a role map that LDS exchanges change. The workgroup
+// interprets the stage's program (vprog.h) on each of its tiles and
+// prefetches the next tile into a second register set while it computes.
+//
+// Coefficients indexed per lane (diagonal tables, scale tables, LDS-op
+// matrices) are copied to LDS once per workgroup; per-instruction uniform
+// coefficients are read through the scalar cache. Inside the tile loop no
+// vector memory access other than the tile's own loads and stores is issued,
+// so waiting on a coefficient never waits on the prefetch.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "vprog.h"
+#include "vstage.h"
+
+namespace qvd {
+
+namespace {
+
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename T2>
+__device__ __forceinline__ T2 mk(decltype(T2::x) a, decltype(T2::x) b) {
+    T2 r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+
+// w * x
+template <typename T2>
+__device__ __forceinline__ T2 cm(const T2 w, const T2 x) {
+    return mk<T2>(fma_(-w.y, x.y, w.x * x.x), fma_(w.y, x.x, w.x * x.y));
+}
+// a + w * x
+template <typename T2>
+__device__ __forceinline__ T2 cfma(const T2 w, const T2 x, const T2 a) {
+    return mk<T2>(fma_(-w.y, x.y, fma_(w.x, x.x, a.x)), fma_(w.y, x.x, fma_(w.x, x.y, a.y)));
+}
+// w0 * x0 + w1 * x1
+template <typename T2>
+__device__ __forceinline__ T2 cdot(const T2 w0, const T2 x0, const T2 w1, const T2 x1) {
+    return cfma(w1, x1, cm(w0, x0));
+}
+
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
+    return make_float2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+__device__ __forceinline__ double2 shfl_xor2(double2 v, int m) {
+    return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+template <typename T2>
+__device__ __forceinline__ T2 sel(bool p, T2 a, T2 b) {
+    return mk<T2>(p ? a.x : b.x, p ? a.y : b.y);
+}
+
+// 1-qubit gate on register bit B.
+template <typename T2, int RB, int B, int FORM, bool CTRL>
+__device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const T2 *__restrict__ c, uint32_t tpred_ok, uint32_t rmask,
+                                    uint32_t rval) {
+    const T2 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    const bool tp = tpred_ok != 0;
+#pragma unroll
+    for (int j0 = 0; j0 < (1 << RB); j0++) {
+        if (j0 & (1 << B)) continue;
+        const int j1 = j0 | (1 << B);
+        if (CTRL && ((uint32_t)j0 & rmask) != rval) continue;
+        const T2 a = x[j0], b = x[j1];
+        T2 o0, o1;
+        if (FORM == FORM_UNIT_DIAG) {
+            o0 = cfma(c1, b, a);
+            o1 = cfma(c2, a, b);
+        } else if (FORM == FORM_UNIT_ANTI) {
+            o0 = cfma(c0, a, b);
+            o1 = cfma(c3, b, a);
+        } else {
+            o0 = cdot(c0, a, c1, b);
+            o1 = cdot(c2, a, c3, b);
+        }
+        if (CTRL) {
+            o0 = sel(tp, o0, a);
+            o1 = sel(tp, o1, b);
+        }
+        x[j0] = o0;
+        x[j1] = o1;
+    }
+}
+
+template <typename T2, int RB, int FORM, bool CTRL>
+__device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const T2 *__restrict__ c, uint32_t tp,
+                                         uint32_t rmask, uint32_t rval) {
+    switch (slot) {
+        case 0: u1r<T2, RB, 0, FORM, CTRL>(x, c, tp, rmask, rval); break;
+        case 1: u1r<T2, RB, 1, FORM, CTRL>(x, c, tp, rmask, rval); break;
+        case 2: u1r<T2, RB, 2, FORM, CTRL>(x, c, tp, rmask, rval); break;
+        case 3: u1r<T2, RB, 3, FORM, CTRL>(x, c, tp, rmask, rval); break;
+        default:
+            if constexpr (RB > 4) u1r<T2, RB, (RB > 4 ? 4 : 0), FORM, CTRL>(x, c, tp, rmask, rval);
+            break;
+    }
+}
+
+// 1-qubit gate on lane bit b: the partner amplitude comes from lane ^ (1 << b).
+template <typename T2, int RB, int FORM, bool CTRL>
+__device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const T2 *__restrict__ c, uint32_t lane,
+                                    uint32_t tp, uint32_t rmask, uint32_t rval) {
+    const bool hi = (lane >> b) & 1;
+    const T2 cs = hi ? c[3] : c[0];  // coefficient of the lane's own amplitude
+    const T2 cp = hi ? c[2] : c[1];  // coefficient of the partner's
+    const int m = 1 << b;
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        if (CTRL && ((uint32_t)j & rmask) != rval) continue;
+        const T2 a = x[j];
+        const T2 p = shfl_xor2(a, m);
+        T2 o;
+        if (FORM == FORM_UNIT_DIAG) o = cfma(cp, p, a);
+        else if (FORM == FORM_UNIT_ANTI) o = cfma(cs, a, p);
+        else o = cdot(cs, a, cp, p);
+        if (CTRL) o = sel(tp != 0, o, a);
+        x[j] = o;
+    }
+}
+
+// Register-index part of a diagonal index: bit j of the result is bit
+// rpos[j] of the register index, for the register targets of the op.
+__device__ __forceinline__ uint32_t reg_index(uint32_t jreg, const uint32_t *rpos, uint32_t rmask_targets) {
+    uint32_t idx = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < MAX_GATE_QUBITS; j++)
+        if ((rmask_targets >> j) & 1) idx |= ((jreg >> rpos[j]) & 1) << j;
+    return idx;
+}
+
+template <typename T2, int RB, bool SIGN>
+__device__ __forceinline__ void diag(T2 (&x)[1 << RB], const VInstr &v, const T2 *__restrict__ cst, uint32_t tid,
+                                     uint64_t base) {
+    uint32_t it = 0, rpos[MAX_GATE_QUBITS], rt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < MAX_GATE_QUBITS; j++) {
+        rpos[j] = 0;
+        if (j >= v.k) continue;
+        const uint32_t e = v.tgt[j], kind = e & VT_KIND, b = e & 0xffffu;
+        if (kind == VT_THREAD) it |= ((tid >> b) & 1) << j;
+        else if (kind == VT_GLOBAL) it |= (uint32_t)((base >> b) & 1) << j;
+        else {
+            rpos[j] = b;
+            rt |= 1u << j;
+        }
+    }
+    const bool tp = (tid & v.tmask) == v.tval;
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        if (((uint32_t)j & v.rmask) != v.rval) continue;
+        const uint32_t idx = it | reg_index((uint32_t)j, rpos, rt);
+        const T2 a = x[j];
+        T2 o;
+        if (SIGN) {
+            const bool neg = (v.neg >> idx) & 1;
+            o = neg ? mk<T2>(-a.x, -a.y) : a;
+        } else {
+            o = cm(cst[v.coef + idx], a);
+        }
+        x[j] = tp ? o : a;
+    }
+}
+
+// In-LDS dense op (the general path): groups striped over the workgroup.
+template <typename T2, int K, int W>
+__device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__restrict__ m, const StageOp &op,
+                                          uint32_t L) {
+    constexpr int D = 1 << K;
+    uint32_t off[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int b = 0; b < K; b++) o |= (uint32_t)((j >> b) & 1) << op.tgt[b];
+        off[j] = o;
+    }
+    const uint32_t groups = 1u << (L - op.nins);
+    for (uint32_t g = threadIdx.x; g < groups; g += W) {
+        uint32_t i0 = g;
+        for (uint32_t j = 0; j < op.nins; j++) {
+            const uint32_t p = op.ins[j];
+            i0 = ((i0 >> p) << (p + 1)) | (i0 & ((1u << p) - 1));
+        }
+        i0 |= op.lc_val;
+        T2 a[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) a[j] = tile[i0 | off[j]];
+        // Rows one at a time (not unrolled): the row's offset is rebuilt
+        // from its bits so no array is indexed at run time.
+#pragma unroll 1
+        for (int r = 0; r < D; r++) {
+            T2 acc = mk<T2>(0, 0);
+#pragma unroll
+            for (int c = 0; c < D; c++) acc = cfma(m[r * D + c], a[c], acc);
+            uint32_t orow = 0;
+#pragma unroll
+            for (int b = 0; b < K; b++) orow |= (uint32_t)((r >> b) & 1) << op.tgt[b];
+            tile[i0 | orow] = acc;
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T2, int WB, int RB>
+__global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc d,
+                                                     const VInstr *__restrict__ prog, const T2 *__restrict__ coefs,
+                                                     const StageOp *__restrict__ ldsops,
+                                                     const uint32_t *__restrict__ ints) {
+    constexpr int W = 64 << WB, R = 1 << RB;
+    constexpr uint32_t L = 6 + WB + RB;
+    extern __shared__ unsigned char smem_raw[];
+    T2 *tile = reinterpret_cast<T2 *>(smem_raw);
+    T2 *cst = tile + (1u << L);  // the stage's coefficients (index = coef - cbegin)
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+
+    for (uint32_t i = tid; i < d.ccount; i += W) cst[i] = coefs[d.cbegin + i];
+    const T2 *cbase = cst - d.cbegin;  // so cbase[v.coef + i] is the LDS copy
+    __syncthreads();
+
+    // Physical offsets of the initial role map (loads) and the final one
+    // (stores): thread part per lane, register part per register bit.
+    uint64_t t_off0 = lane, t_offF = lane;
+#pragma unroll
+    for (int s_ = 0; s_ < WB; s_++) {
+        t_off0 |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[d.wmap0[s_]];
+        t_offF |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[d.wmapF[s_]];
+    }
+    uint64_t p0[RB], pF[RB];
+#pragma unroll
+    for (int b = 0; b < RB; b++) {
+        p0[b] = uint64_t(1) << d.local_pos[d.rmap0[b]];
+        pF[b] = uint64_t(1) << d.local_pos[d.rmapF[b]];
+    }
+
+    T2 x[R], y[R];
+    auto tile_base = [&](uint64_t t) {
+        uint64_t base = 0;
+        for (uint32_t j = 0; j < d.nglob; j++) base |= ((t >> j) & 1) << d.glob_pos[j];
+        return base;
+    };
+    auto load = [&](T2(&dst)[R], uint64_t base) {
+        const T2 *src = s + base + t_off0;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+                if ((j >> b) & 1) o += p0[b];
+            dst[j] = src[o];
+        }
+    };
+
+    uint64_t t = blockIdx.x;
+    if (t < d.num_tiles) load(y, tile_base(t));
+    for (; t < d.num_tiles; t += gridDim.x) {
+        const uint64_t base = tile_base(t);
+#pragma unroll
+        for (int j = 0; j < R; j++) x[j] = y[j];
+        const uint64_t tn = t + gridDim.x;
+        if (tn < d.num_tiles) load(y, tile_base(tn));
+
+        // Current role map: tile-index contributions of the wave bits (per
+        // thread) and of the register bits (uniform).
+        uint32_t wm[WB], rm[RB];
+#pragma unroll
+        for (int s_ = 0; s_ < WB; s_++) wm[s_] = d.wmap0[s_];
+#pragma unroll
+        for (int b = 0; b < RB; b++) rm[b] = d.rmap0[b];
+
+        for (uint32_t ii = 0; ii < d.num_instr; ii++) {
+            const VInstr &v = prog[d.instr_off + ii];
+            if ((base & v.gmask) != v.gval) continue;
+            const uint32_t tp = (tid & v.tmask) == v.tval;
+            const bool ctrl = v.tmask != 0 || v.rmask != 0;
+            switch (v.op) {
+                case V_U1R: {
+                    const T2 *c = coefs + v.coef;
+                    if (!ctrl) {
+                        if (v.form == FORM_UNIT_DIAG) u1r_slot<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, 1, 0, 0);
+                        else if (v.form == FORM_UNIT_ANTI) u1r_slot<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, 1, 0, 0);
+                        else u1r_slot<T2, RB, FORM_FULL, false>(x, v.slot, c, 1, 0, 0);
+                    } else {
+                        u1r_slot<T2, RB, FORM_FULL, true>(x, v.slot, c, tp, v.rmask, v.rval);
+                    }
+                    break;
+                }
+                case V_U1L: {
+                    const T2 *c = coefs + v.coef;
+                    if (!ctrl) {
+                        if (v.form == FORM_UNIT_DIAG) u1l<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, lane, 1, 0, 0);
+                        else if (v.form == FORM_UNIT_ANTI) u1l<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, lane, 1, 0, 0);
+                        else u1l<T2, RB, FORM_FULL, false>(x, v.slot, c, lane, 1, 0, 0);
+                    } else {
+                        u1l<T2, RB, FORM_FULL, true>(x, v.slot, c, lane, tp, v.rmask, v.rval);
+                    }
+                    break;
+                }
+                case V_DIAG:
+                    if (v.form == 1) diag<T2, RB, true>(x, v, cbase, tid, base);
+                    else diag<T2, RB, false>(x, v, cbase, tid, base);
+                    break;
+                case V_SCALE: {
+                    if (v.form == 1) {
+                        const T2 f = coefs[v.coef];
+#pragma unroll
+                        for (int j = 0; j < R; j++) x[j] = cm(f, x[j]);
+                    } else {
+                        T2 ft = cbase[v.coef + tid];
+                        const T2 *g = cbase + v.coef + W + R;
+                        for (uint32_t i = 0; i < v.k; i++) {
+                            const uint32_t q = ints[v.aux + i];
+                            ft = cm(g[2 * i + ((base >> q) & 1)], ft);
+                        }
+#pragma unroll
+                        for (int j = 0; j < R; j++) x[j] = cm(cm(coefs[v.coef + W + j], ft), x[j]);
+                    }
+                    break;
+                }
+                case V_XCHG:
+                case V_LDSOP: {
+                    uint32_t ti = lane;
+#pragma unroll
+                    for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << wm[s_];
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < R; j++) {
+                        uint32_t o = 0;
+#pragma unroll
+                        for (int b = 0; b < RB; b++)
+                            if ((j >> b) & 1) o |= 1u << rm[b];
+                        tile[ti | o] = x[j];
+                    }
+                    __syncthreads();
+                    if (v.op == V_LDSOP) {
+                        const StageOp &op = ldsops[v.aux];
+                        const T2 *m = cbase + op.coef_off;
+                        switch (op.k) {
+                            case 2: lds_dense<T2, 2, W>(tile, m, op, L); break;
+                            case 3: lds_dense<T2, 3, W>(tile, m, op, L); break;
+                            case 4: lds_dense<T2, 4, W>(tile, m, op, L); break;
+                            default: lds_dense<T2, 1, W>(tile, m, op, L); break;
+                        }
+                        __syncthreads();
+                    } else {
+#pragma unroll
+                        for (int s_ = 0; s_ < WB; s_++) wm[s_] = v.wmap[s_];
+#pragma unroll
+                        for (int b = 0; b < RB; b++) rm[b] = v.rmap[b];
+                        ti = lane;
+#pragma unroll
+                        for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << wm[s_];
+                    }
+#pragma unroll
+                    for (int j = 0; j < R; j++) {
+                        uint32_t o = 0;
+#pragma unroll
+                        for (int b = 0; b < RB; b++)
+                            if ((j >> b) & 1) o |= 1u << rm[b];
+                        x[j] = tile[ti | o];
+                    }
+                    break;
+                }
+                default: break;
+            }
+        }
+
+        T2 *dst = s + base + t_offF;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int b = 0; b < RB; b++)
+                if ((j >> b) & 1) o += pF[b];
+            dst[o] = x[j];
+        }
+        // The next tile's first exchange writes LDS only after a barrier, so
+        // no barrier is needed here.
+    }
+}
+
+template <typename T2>
+VGeom vstage_geom() {
+    return sizeof(T2) == 8 ? VGeom{3, 5} : VGeom{3, 4};
+}
+
+template <typename T2>
+size_t vstage_lds_bytes() {
+    const VGeom g = vstage_geom<T2>();
+    return (sizeof(T2) << g.L()) + VCONST_BYTES;
+}
+
+template <typename T2>
+hipError_t launch_vstage(T2 *s, const VStageDesc &d, const VInstr *prog, const T2 *coefs, const StageOp *ldsops,
+                         const uint32_t *ints, unsigned grid, hipStream_t st) {
+    const size_t lds = vstage_lds_bytes<T2>();
+    if constexpr (sizeof(T2) == 8) {
+        auto k = k_vstage<T2, 3, 5>;
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        k<<<grid, 64 << 3, lds, st>>>(s, d, prog, coefs, ldsops, ints);
+    } else {
+        auto k = k_vstage<T2, 3, 4>;
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        k<<<grid, 64 << 3, lds, st>>>(s, d, prog, coefs, ldsops, ints);
+    }
+    return hipGetLastError();
+}
+
+template VGeom vstage_geom<float2>();
+template VGeom vstage_geom<double2>();
+template size_t vstage_lds_bytes<float2>();
+template size_t vstage_lds_bytes<double2>();
+template hipError_t launch_vstage<float2>(float2 *, const VStageDesc &, const VInstr *, const float2 *,
+                                          const StageOp *, const uint32_t *, unsigned, hipStream_t);
+template hipError_t launch_vstage<double2>(double2 *, const VStageDesc &, const VInstr *, const double2 *,
+                                           const StageOp *, const uint32_t *, unsigned, hipStream_t);
+
+}  // namespace qvd

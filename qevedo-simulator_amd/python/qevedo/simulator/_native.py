@@ -24,7 +24,7 @@ FLAG_TIMING = 1
 EXPORTS = (
     "qvd_last_error", "qvd_device_info", "qvd_sv_create", "qvd_sv_destroy", "qvd_sv_num_qubits",
     "qvd_sv_precision", "qvd_sv_set_zero", "qvd_sv_set_amplitudes", "qvd_sv_get_amplitudes", "qvd_sv_apply",
-    "qvd_sv_apply_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
+    "qvd_sv_apply_ops", "qvd_plan_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
     "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_compare", "qvd_sv_last_sample_seconds",
     "qvd_gate_matrix", "qvd_gate_lower",
 )
@@ -104,6 +104,7 @@ def lib():
         L.qvd_sv_measure.argtypes = [vp, u32, d, P(C.c_uint8)]
         L.qvd_sv_reset.argtypes = [vp, u32, d]
         L.qvd_sv_sample.argtypes = [vp, P(d), u64, P(u64)]
+        L.qvd_plan_ops.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(Stats)]
         L.qvd_sv_compare.argtypes = [vp, vp, P(d), P(d), P(d)]
         L.qvd_sv_last_sample_seconds.argtypes = [vp, P(d)]
         L.qvd_gate_matrix.argtypes = [C.c_char_p, P(d), u32, P(d), u32, P(u32)]
@@ -179,6 +180,16 @@ class OpArray:
                 raise ValueError(f"a gate on {t.size} qubits needs a {1 << t.size}x{1 << t.size} matrix")
             self.keep += [t, cq, cv, m]
             self.arr[i] = Op(_p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size, _p(m.view(np.float64), C.c_double))
+
+
+def plan_ops(num_qubits: int, ops, precision: str = "single", max_fused_qubits: int = 4, tile_bits: int | None = 0,
+             planner: int = PLANNER_HEURISTIC) -> dict:
+    """Plan statistics (stages, passes, fused ops) of apply_ops, on the host only."""
+    arr = ops if isinstance(ops, OpArray) else OpArray(ops)
+    opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0)
+    st = Stats()
+    _check(lib().qvd_plan_ops(num_qubits, {"single": C64, "double": C128}[precision], arr.arr, arr.n, C.byref(opt), C.byref(st)))
+    return st.as_dict()
 
 
 class StateVector:

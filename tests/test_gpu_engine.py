@@ -124,9 +124,11 @@ def test_library_circuits():
             st = s.apply_ops(lower_gates(gates), 4, 0)
             assert max_distance(s.amplitudes(), expected) < 1e-12
             assert st["kernels"] < st["gates"]
+    # Fusion cuts kernel calls on the fused-op (LDS tile) path; the register
+    # tiles apply unfused gates, so this is checked with 13-qubit LDS tiles.
     gates = library.random_circuit(20, 20, 1)
     with StateVector(20, "single") as s:
-        st = s.apply_ops(lower_gates(gates), 4, 0)
+        st = s.apply_ops(lower_gates(gates), 4, 13)
     assert st["gates"] > 500 and st["kernels"] * 4 < st["gates"]
 
 
@@ -146,6 +148,45 @@ def test_random_circuit_against_blocked_fp64(n):
     with StateVector(n, "double") as s:
         s.apply_ops(lower_gates(gates), 4, 0)
         assert max_distance(s.amplitudes(), want) < 1e-11
+
+
+@pytest.mark.parametrize("precision,tol", [("single", 1e-5), ("double", 1e-11)])
+def test_register_tiles_every_gate_kind(precision, tol):
+    """States of at least one register tile (14 qubits in complex64, 13 in
+    complex128) run the register-tile kernel: random circuits over the whole
+    gate set exercise register-bit and lane-bit 1-qubit gates, controls on
+    lane / wave / register / non-tile bits, diagonals, LDS exchanges and the
+    in-LDS path for SWAP, CCX and CSWAP."""
+    _note(f"register tiles {precision}")
+    for seed in range(12):
+        n = (14 if precision == "single" else 13) + seed % 4
+        gates = library.random_gate_circuit(n, 150, 500 + seed)
+        expected = oracle_statevector(n, gates)
+        with StateVector(n, precision) as s:
+            st = s.apply_ops(lower_gates(gates))
+            got = s.amplitudes()
+        assert max_distance(got, expected) < tol, (seed, n, max_distance(got, expected))
+        assert st["stages"] >= 1
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_register_tiles_structured_circuits(precision):
+    """Kicked Ising on a heavy-hex graph, a hardware-efficient ansatz and the
+    random circuit family on register tiles, against the FP64 oracle."""
+    _note(f"structured circuits {precision}")
+    n = 16
+    circuits = [
+        library.kicked_ising(n, library.heavy_hex_edges(n), 5, 0.9),
+        library.hardware_efficient_ansatz(n, 3, 4),
+        library.random_circuit(n, 14, 11),
+        library.qft(n),
+    ]
+    tol = 1e-5 if precision == "single" else 1e-11
+    for gates in circuits:
+        expected = oracle_statevector(n, gates)
+        with StateVector(n, precision) as s:
+            s.apply_ops(lower_gates(gates))
+            assert max_distance(s.amplitudes(), expected) < tol
 
 
 def test_qft_closed_form():
