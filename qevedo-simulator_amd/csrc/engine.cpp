@@ -410,10 +410,10 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
     p.max_fused = std::max<uint32_t>(opt.max_fused_qubits, 1);
     p.max_stage_op = 4;
     if (opt.tile_bits == 0 && n >= vg.L()) {
-        // Register-tile stages: lane bits are qubits 0..5.
+        // Register-tile stages: the fixed lane bits are the lowest qubits.
         p.reg_tile = true;
         p.tile_bits = vg.L();
-        p.contig = 6;
+        p.contig = vg.f_bits;
     } else {
         p.tile_bits = opt.tile_bits > 0 ? (uint32_t)opt.tile_bits : tc.L;
         require(p.tile_bits <= 14 - (precision == QVD_C128 ? 1 : 0) && p.tile_bits >= 1,

@@ -2,9 +2,9 @@
 //
 // The program generator orders a stage's ops by their true dependencies
 // (the dependency-aware extraction of src/blocking.rs:157-179, applied to
-// the register bits of a tile instead of the local qubits of a region),
-// inserts LDS exchanges when a gate needs a wave bit, merges runs of
-// 1-qubit gates on one qubit, and keeps 1-qubit gates in unit form with
+// the lane and register bits of a tile instead of the local qubits of a
+// region), inserts LDS exchanges when a gate needs a wave bit, merges runs
+// of 1-qubit gates on one qubit, and keeps 1-qubit gates in unit form with
 // their diagonal factors carried forward.
 #include "vprog.h"
 
@@ -18,19 +18,28 @@ namespace {
 
 enum Role : uint8_t { ROLE_LANE, ROLE_WAVE, ROLE_REG, ROLE_NONE };
 
+// Which tile bit each flexible lane / wave / register slot holds.
 struct Roles {
-    std::vector<uint8_t> wmap, rmap;  // wave / register slot -> tile bit
-    std::vector<Role> role;           // tile bit -> role
-    std::vector<uint8_t> slot;        // tile bit -> slot index
+    uint32_t F = 0;
+    std::vector<uint8_t> lmap, wmap, rmap;
+    std::vector<Role> role;     // tile bit -> role
+    std::vector<uint8_t> slot;  // tile bit -> lane bit (lanes), wave slot, register slot
 
-    void assign(uint32_t L, const std::vector<uint8_t> &w, const std::vector<uint8_t> &r) {
+    void assign(uint32_t L, uint32_t f, const std::vector<uint8_t> &l, const std::vector<uint8_t> &w,
+                const std::vector<uint8_t> &r) {
+        F = f;
+        lmap = l;
         wmap = w;
         rmap = r;
         role.assign(L, ROLE_NONE);
         slot.assign(L, 0);
-        for (uint32_t b = 0; b < 6; b++) {
+        for (uint32_t b = 0; b < F; b++) {
             role[b] = ROLE_LANE;
             slot[b] = (uint8_t)b;
+        }
+        for (size_t s = 0; s < l.size(); s++) {
+            role[l[s]] = ROLE_LANE;
+            slot[l[s]] = (uint8_t)(F + s);
         }
         for (size_t s = 0; s < w.size(); s++) {
             role[w[s]] = ROLE_WAVE;
@@ -48,6 +57,23 @@ using M2 = std::array<C64, 4>;  // row-major 2x2
 M2 mul2(const M2 &a, const M2 &b) {
     return {cmul(a[0], b[0]) + cmul(a[1], b[2]), cmul(a[0], b[1]) + cmul(a[1], b[3]),
             cmul(a[2], b[0]) + cmul(a[3], b[2]), cmul(a[2], b[1]) + cmul(a[3], b[3])};
+}
+
+bool has(const std::vector<uint8_t> &v, uint32_t x) { return std::find(v.begin(), v.end(), (uint8_t)x) != v.end(); }
+
+// Keeps each bit in its old slot of the same role where it can, so only the
+// bits that move change slots.
+std::vector<uint8_t> stabilize(const std::vector<uint8_t> &want, const std::vector<uint8_t> &old) {
+    std::vector<uint8_t> out(want.size(), 0xff), rest;
+    for (uint8_t b : want) {
+        auto it = std::find(old.begin(), old.end(), b);
+        if (it != old.end() && (size_t)(it - old.begin()) < out.size()) out[it - old.begin()] = b;
+        else rest.push_back(b);
+    }
+    size_t k = 0;
+    for (auto &s : out)
+        if (s == 0xff) s = rest[k++];
+    return out;
 }
 
 struct Gen {
@@ -72,12 +98,12 @@ struct Gen {
         for (size_t j = 0; j < s.local.size(); j++) tbit[s.local[j]] = (int)j;
     }
 
-    // Where a physical qubit's bit lives now, as a DIAG target encoding.
+    // Where a physical qubit's bit lives now.
     uint32_t encode(uint32_t q) const {
         const int b = tbit[q];
         if (b < 0) return VT_GLOBAL | q;
         switch (roles.role[b]) {
-            case ROLE_LANE: return VT_THREAD | (uint32_t)b;
+            case ROLE_LANE: return VT_THREAD | roles.slot[b];
             case ROLE_WAVE: return VT_THREAD | (6u + roles.slot[b]);
             default: return VT_REG | roles.slot[b];
         }
@@ -109,12 +135,12 @@ struct Gen {
         return v;
     }
 
-    // Emits a 1-qubit gate m (row-major, in the current frame) on q.
+    // Emits a 1-qubit gate m (row-major) on q.
     void emit_u1(uint32_t q, const M2 &m, uint32_t form, const std::vector<std::pair<uint32_t, bool>> &controls) {
         const int b = tbit[q];
         require(b >= 0 && roles.role[b] != ROLE_WAVE, "internal: 1-qubit gate on a wave bit");
         VInstr v = blank(roles.role[b] == ROLE_LANE ? V_U1L : V_U1R);
-        v.slot = roles.role[b] == ROLE_LANE ? (uint32_t)b : roles.slot[b];
+        v.slot = roles.slot[b];
         v.form = form;
         add_controls(v, controls);
         v.coef = (uint32_t)coefs.size();
@@ -169,8 +195,15 @@ struct Gen {
         for (uint32_t q : op.qubits()) flush_pmat(q);
         VInstr v = blank(V_DIAG);
         v.k = (uint32_t)op.targets.size();
-        for (uint32_t j = 0; j < v.k; j++) v.tgt[j] = encode(op.targets[j]);
+        for (uint32_t j = 0; j < v.k; j++) {
+            v.tgt[j] = encode(op.targets[j]);
+            const uint32_t x = v.tgt[j] & 0xffffu;
+            if ((v.tgt[j] & VT_KIND) == VT_THREAD) v.threaded = 1;
+            if ((v.tgt[j] & VT_KIND) == VT_REG)
+                for (uint32_t r = 0; r < (1u << g.r_bits); r++) v.rtab[r] |= (uint8_t)(((r >> x) & 1) << j);
+        }
         add_controls(v, op.controls);
+        if (v.tmask) v.threaded = 1;
         v.coef = (uint32_t)coefs.size();
         bool sign = true;
         uint64_t neg = 0;
@@ -270,21 +303,17 @@ struct Gen {
         }
     }
 
-    // Tile bits an op needs outside the wave bits.
-    void needs(const Op &op, std::vector<uint32_t> &outb) const {
-        outb.clear();
-        if (op.diagonal) return;
-        if (op.targets.size() != 1) return;  // multi-qubit dense: applied in LDS
+    // The flexible tile bit a 1-qubit gate needs as a lane or register bit
+    // (none for diagonals and multi-qubit dense ops, which run anywhere).
+    int need(const Op &op) const {
+        if (op.diagonal || op.targets.size() != 1) return -1;
         const int b = tbit[op.targets[0]];
-        if (b >= 6) outb.push_back((uint32_t)b);
+        return b >= (int)g.f_bits ? b : -1;
     }
 
     bool runnable(const Op &op) const {
-        std::vector<uint32_t> nb;
-        needs(op, nb);
-        for (uint32_t b : nb)
-            if (roles.role[b] == ROLE_WAVE) return false;
-        return true;
+        const int b = need(op);
+        return b < 0 || roles.role[b] != ROLE_WAVE;
     }
 
     template <typename F>
@@ -306,61 +335,44 @@ struct Gen {
         return taken;
     }
 
-    // The register bits the coming ops want, in order of need (capacity r).
-    std::vector<uint32_t> lookahead(const std::vector<char> &done) const {
-        std::vector<uint32_t> wanted, nb;
+    // The flexible bits the coming gates want as lane or register bits, in
+    // order of need, up to the number of such slots.
+    std::vector<uint8_t> lookahead(const std::vector<char> &done) const {
+        const size_t cap = g.r_bits + g.flex_lanes();
+        std::vector<uint8_t> wanted;
         extract(done, [&](const Op &op) {
-            needs(op, nb);
-            std::vector<uint32_t> fresh;
-            for (uint32_t b : nb)
-                if (!contains(wanted, b)) fresh.push_back(b);
-            if (wanted.size() + fresh.size() > g.r_bits) return false;
-            wanted.insert(wanted.end(), fresh.begin(), fresh.end());
+            const int b = need(op);
+            if (b < 0 || has(wanted, (uint32_t)b)) return true;
+            if (wanted.size() + 1 > cap) return false;
+            wanted.push_back((uint8_t)b);
             return true;
         });
         return wanted;
     }
 
-    // New role map: wanted bits (and then current register bits) as
-    // register bits, the rest as wave bits.
-    void choose_roles(const std::vector<uint32_t> &wanted, bool initial, std::vector<uint8_t> &w,
-                      std::vector<uint8_t> &r) const {
-        const uint32_t L = (uint32_t)st.local.size();
-        r.clear();
-        w.clear();
-        for (uint32_t b : wanted)
-            if (r.size() < g.r_bits) r.push_back((uint8_t)b);
-        if (!initial)
-            for (uint8_t b : roles.rmap)
-                if (r.size() < g.r_bits && !contains(std::vector<uint32_t>(r.begin(), r.end()), b)) r.push_back(b);
-        for (uint32_t b = 6; b < L && r.size() < g.r_bits; b++)
-            if (!contains(std::vector<uint32_t>(r.begin(), r.end()), b)) r.push_back((uint8_t)b);
-        for (uint32_t b = 6; b < L; b++)
-            if (!contains(std::vector<uint32_t>(r.begin(), r.end()), b)) w.push_back((uint8_t)b);
-        // Keep register slots of bits that stay registers, so only moved bits
-        // change places (fewer instructions depend on the slot order).
+    // New role map: wanted bits first as register bits, then as lane bits;
+    // current register and lane bits fill the rest of those slots; the
+    // remaining flexible bits become wave bits.
+    void choose_roles(const std::vector<uint8_t> &wanted, bool initial, std::vector<uint8_t> &l,
+                      std::vector<uint8_t> &w, std::vector<uint8_t> &r) const {
+        const uint32_t L = (uint32_t)st.local.size(), F = g.f_bits;
+        std::vector<uint8_t> order = wanted;
         if (!initial) {
-            std::vector<uint8_t> keep(g.r_bits, 0xff);
-            std::vector<uint8_t> rest;
-            for (uint8_t b : r) {
-                auto it = std::find(roles.rmap.begin(), roles.rmap.end(), b);
-                if (it != roles.rmap.end()) keep[it - roles.rmap.begin()] = b;
-                else rest.push_back(b);
-            }
-            size_t k = 0;
-            for (auto &s : keep)
-                if (s == 0xff) s = rest[k++];
-            r = keep;
-            std::vector<uint8_t> wkeep(g.w_bits, 0xff), wrest;
-            for (uint8_t b : w) {
-                auto it = std::find(roles.wmap.begin(), roles.wmap.end(), b);
-                if (it != roles.wmap.end()) wkeep[it - roles.wmap.begin()] = b;
-                else wrest.push_back(b);
-            }
-            k = 0;
-            for (auto &s : wkeep)
-                if (s == 0xff) s = wrest[k++];
-            w = wkeep;
+            for (uint8_t b : roles.rmap)
+                if (!has(order, b)) order.push_back(b);
+            for (uint8_t b : roles.lmap)
+                if (!has(order, b)) order.push_back(b);
+        }
+        for (uint32_t b = F; b < L; b++)
+            if (!has(order, b)) order.push_back((uint8_t)b);
+        r.assign(order.begin(), order.begin() + g.r_bits);
+        l.assign(order.begin() + g.r_bits, order.begin() + g.r_bits + g.flex_lanes());
+        w.assign(order.begin() + g.r_bits + g.flex_lanes(), order.end());
+        if (!initial) {
+            // A wanted bit already in a lane slot can stay there.
+            r = stabilize(r, roles.rmap);
+            l = stabilize(l, roles.lmap);
+            w = stabilize(w, roles.wmap);
         }
     }
 
@@ -406,15 +418,16 @@ struct Gen {
     VProgram run(bool flush_all) {
         const uint32_t L = (uint32_t)st.local.size();
         require(L == g.L(), "internal: stage tile size does not match the register tile");
-        for (uint32_t b = 0; b < 6; b++) require(st.local[b] == b, "internal: lane bits must be qubits 0..5");
+        for (uint32_t b = 0; b < g.f_bits; b++) require(st.local[b] == b, "internal: fixed lane bits must be the lowest qubits");
         std::vector<char> done(st.ops.size(), 0);
-        std::vector<uint8_t> w, r;
-        choose_roles(lookahead(done), true, w, r);
-        roles.assign(L, w, r);
+        std::vector<uint8_t> l, w, r;
+        choose_roles(lookahead(done), true, l, w, r);
+        roles.assign(L, g.f_bits, l, w, r);
         VProgram p;
         p.desc = VStageDesc{};
-        for (uint32_t s = 0; s < g.w_bits; s++) p.desc.wmap0[s] = roles.wmap[s];
-        for (uint32_t s = 0; s < g.r_bits; s++) p.desc.rmap0[s] = roles.rmap[s];
+        for (size_t s = 0; s < l.size(); s++) p.desc.lmap0[s] = l[s];
+        for (size_t s = 0; s < w.size(); s++) p.desc.wmap0[s] = w[s];
+        for (size_t s = 0; s < r.size(); s++) p.desc.rmap0[s] = r[s];
         size_t remaining = st.ops.size();
         while (remaining) {
             std::vector<size_t> taken = extract(done, [&](const Op &op) { return runnable(op); });
@@ -424,23 +437,23 @@ struct Gen {
             }
             remaining -= taken.size();
             if (!remaining) break;
-            // Exchange: the wanted bits become register bits. Pending
-            // products on bits that change role are flushed first, since
-            // their slot is about to move.
-            std::vector<uint32_t> wanted = lookahead(done);
-            choose_roles(wanted, false, w, r);
+            // Exchange so the wanted bits become lane or register bits.
+            // Pending products are emitted first, in the slots they have now.
+            choose_roles(lookahead(done), false, l, w, r);
             for (uint32_t q = 0; q < n; q++) flush_pmat(q);
+            require(l != roles.lmap || w != roles.wmap || r != roles.rmap, "internal: exchange without progress");
             VInstr v = blank(V_XCHG);
-            for (uint32_t s = 0; s < g.w_bits; s++) v.wmap[s] = w[s];
-            for (uint32_t s = 0; s < g.r_bits; s++) v.rmap[s] = r[s];
-            require(w != roles.wmap || r != roles.rmap, "internal: exchange without progress");
-            roles.assign(L, w, r);
+            for (size_t s = 0; s < l.size(); s++) v.lmap[s] = l[s];
+            for (size_t s = 0; s < w.size(); s++) v.wmap[s] = w[s];
+            for (size_t s = 0; s < r.size(); s++) v.rmap[s] = r[s];
+            roles.assign(L, g.f_bits, l, w, r);
             out.push_back(v);
         }
         for (uint32_t q = 0; q < n; q++) flush_pmat(q);
         if (flush_all) emit_scale();
-        for (uint32_t s = 0; s < g.w_bits; s++) p.desc.wmapF[s] = roles.wmap[s];
-        for (uint32_t s = 0; s < g.r_bits; s++) p.desc.rmapF[s] = roles.rmap[s];
+        for (size_t s = 0; s < roles.lmap.size(); s++) p.desc.lmapF[s] = roles.lmap[s];
+        for (size_t s = 0; s < roles.wmap.size(); s++) p.desc.wmapF[s] = roles.wmap[s];
+        for (size_t s = 0; s < roles.rmap.size(); s++) p.desc.rmapF[s] = roles.rmap[s];
         p.instr = std::move(out);
         return p;
     }

@@ -3,11 +3,12 @@
 // A register-tile stage keeps its tile of 2^L amplitudes in VGPRs: thread t
 // of a workgroup of W = 64 * 2^w threads holds R = 2^r amplitudes, and
 // L = 6 + w + r. Tile bits play three roles:
-//   - lane bits: tile bits 0..5 (physical qubits 0..5, so every wave-wide
-//     access is 64 consecutive amplitudes); gates on them use cross-lane
-//     shuffles;
+//   - lane bits (the 6 low bits of the thread index): the lowest F of them
+//     are tile bits 0..F-1, physical qubits 0..F-1, so every access is a run
+//     of 2^F consecutive amplitudes (128 bytes); the other 6 - F lane bits
+//     hold any tile bits. Gates on lane bits use cross-lane shuffles;
 //   - wave bits: the remaining bits of the thread index; a gate on one needs
-//     an exchange through LDS that makes it a register bit;
+//     an exchange through LDS that makes it a lane or register bit;
 //   - register bits: the index into the thread's registers; gates on them
 //     are thread-local arithmetic.
 // A program is a list of VInstr the kernel interprets in order; every
@@ -45,6 +46,8 @@ constexpr uint32_t VT_KIND = 3u << 16;
 
 constexpr uint32_t VMAX_W_BITS = 4;
 constexpr uint32_t VMAX_R_BITS = 6;
+constexpr uint32_t VMAX_L_FLEX = 4;  // flexible lane bits
+constexpr uint32_t VMAX_REGS = 32;
 
 struct VInstr {
     uint32_t op;
@@ -58,9 +61,12 @@ struct VInstr {
     uint32_t coef;         // first coefficient (complex) of the op
     uint32_t aux;          // LDSOP: index of its StageOp; SCALE: first non-tile qubit in the int array
     uint32_t tgt[MAX_GATE_QUBITS];            // DIAG targets (VT_* | bit)
+    uint32_t threaded;                        // DIAG: some target or control is a thread-index bit
+    uint8_t rtab[VMAX_REGS];                  // DIAG: register-index part of the entry index, per register
+    uint8_t lmap[VMAX_L_FLEX];                // XCHG: tile bit of each flexible lane bit
     uint8_t wmap[VMAX_W_BITS];                // XCHG: tile bit of each wave bit
     uint8_t rmap[VMAX_R_BITS];                // XCHG: tile bit of each register bit
-    uint8_t pad[6];
+    uint8_t pad[2];
 };
 
 // One register-tile stage as launched.
@@ -72,9 +78,11 @@ struct VStageDesc {
     uint64_t num_tiles;
     uint32_t local_pos[MAX_QUBITS];  // tile bit -> physical qubit (ascending)
     uint32_t glob_pos[MAX_QUBITS];   // tile-index bit -> physical qubit
-    uint8_t wmap0[VMAX_W_BITS];      // role map at load
+    uint8_t lmap0[VMAX_L_FLEX];      // role map at load
+    uint8_t wmap0[VMAX_W_BITS];
     uint8_t rmap0[VMAX_R_BITS];
-    uint8_t wmapF[VMAX_W_BITS];      // role map at store
+    uint8_t lmapF[VMAX_L_FLEX];      // role map at store
+    uint8_t wmapF[VMAX_W_BITS];
     uint8_t rmapF[VMAX_R_BITS];
     uint32_t cbegin, ccount;         // the stage's coefficients, copied to LDS once per workgroup
 };
@@ -86,7 +94,9 @@ constexpr uint32_t VCONST_BYTES = 24 * 1024;
 struct VGeom {
     uint32_t w_bits;  // wave bits (W = 64 << w_bits threads)
     uint32_t r_bits;  // register bits
+    uint32_t f_bits;  // fixed lane bits (tile bits 0..F-1 = physical qubits 0..F-1)
     uint32_t L() const { return 6 + w_bits + r_bits; }
+    uint32_t flex_lanes() const { return 6 - f_bits; }
 };
 
 // Per-qubit diagonal factors not yet applied to the amplitudes: the engine

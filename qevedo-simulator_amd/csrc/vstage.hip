@@ -130,46 +130,38 @@ __device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const T2 *__re
     }
 }
 
-// Register-index part of a diagonal index: bit j of the result is bit
-// rpos[j] of the register index, for the register targets of the op.
-__device__ __forceinline__ uint32_t reg_index(uint32_t jreg, const uint32_t *rpos, uint32_t rmask_targets) {
-    uint32_t idx = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < MAX_GATE_QUBITS; j++)
-        if ((rmask_targets >> j) & 1) idx |= ((jreg >> rpos[j]) & 1) << j;
-    return idx;
-}
-
-template <typename T2, int RB, bool SIGN>
+// A diagonal op: entry index = (thread-index part, per lane) | (non-tile
+// part, per tile) | rtab[j] (register part, precomputed per register).
+// THREADED = false when no target or control is a thread-index bit: then
+// every decision is uniform and sign flips are plain branches.
+template <typename T2, int RB, bool SIGN, bool THREADED>
 __device__ __forceinline__ void diag(T2 (&x)[1 << RB], const VInstr &v, const T2 *__restrict__ cst, uint32_t tid,
                                      uint64_t base) {
-    uint32_t it = 0, rpos[MAX_GATE_QUBITS], rt = 0;
+    uint32_t it = 0, ig = 0;
 #pragma unroll
     for (uint32_t j = 0; j < MAX_GATE_QUBITS; j++) {
-        rpos[j] = 0;
         if (j >= v.k) continue;
         const uint32_t e = v.tgt[j], kind = e & VT_KIND, b = e & 0xffffu;
-        if (kind == VT_THREAD) it |= ((tid >> b) & 1) << j;
-        else if (kind == VT_GLOBAL) it |= (uint32_t)((base >> b) & 1) << j;
-        else {
-            rpos[j] = b;
-            rt |= 1u << j;
-        }
+        if (THREADED && kind == VT_THREAD) it |= ((tid >> b) & 1) << j;
+        else if (kind == VT_GLOBAL) ig |= (uint32_t)((base >> b) & 1) << j;
     }
-    const bool tp = (tid & v.tmask) == v.tval;
+    const bool tp = !THREADED || (tid & v.tmask) == v.tval;
 #pragma unroll
     for (int j = 0; j < (1 << RB); j++) {
         if (((uint32_t)j & v.rmask) != v.rval) continue;
-        const uint32_t idx = it | reg_index((uint32_t)j, rpos, rt);
+        const uint32_t ij = ig | v.rtab[j];
         const T2 a = x[j];
-        T2 o;
         if (SIGN) {
-            const bool neg = (v.neg >> idx) & 1;
-            o = neg ? mk<T2>(-a.x, -a.y) : a;
+            if (THREADED) {
+                const bool neg = tp && ((v.neg >> (ij | it)) & 1);
+                x[j] = neg ? mk<T2>(-a.x, -a.y) : a;
+            } else if ((v.neg >> ij) & 1) {
+                x[j] = mk<T2>(-a.x, -a.y);
+            }
         } else {
-            o = cm(cst[v.coef + idx], a);
+            const T2 o = cm(cst[v.coef + (ij | it)], a);
+            x[j] = tp ? o : a;
         }
-        x[j] = tp ? o : a;
     }
 }
 
@@ -214,7 +206,7 @@ __device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__res
 
 }  // namespace
 
-template <typename T2, int WB, int RB>
+template <typename T2, int WB, int RB, int FB>
 __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc d,
                                                      const VInstr *__restrict__ prog, const T2 *__restrict__ coefs,
                                                      const StageOp *__restrict__ ldsops,
@@ -232,7 +224,13 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
 
     // Physical offsets of the initial role map (loads) and the final one
     // (stores): thread part per lane, register part per register bit.
-    uint64_t t_off0 = lane, t_offF = lane;
+    constexpr uint32_t FL = 6 - FB;  // flexible lane bits
+    uint64_t t_off0 = lane & ((1u << FB) - 1), t_offF = t_off0;
+#pragma unroll
+    for (int s_ = 0; s_ < (int)FL; s_++) {
+        t_off0 |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[d.lmap0[s_]];
+        t_offF |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[d.lmapF[s_]];
+    }
 #pragma unroll
     for (int s_ = 0; s_ < WB; s_++) {
         t_off0 |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[d.wmap0[s_]];
@@ -272,11 +270,19 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         const uint64_t tn = t + gridDim.x;
         if (tn < d.num_tiles) load(y, tile_base(tn));
 
-        // Current role map: tile-index contributions of the wave bits (per
-        // thread) and of the register bits (uniform).
-        uint32_t wm[WB], rm[RB];
+        // Current role map: the thread's part of the tile index (fixed lane
+        // bits, flexible lane bits, wave bits) and the register bits' tile
+        // bits (uniform).
+        auto thread_index = [&](const uint8_t *lm, const uint8_t *wmp) {
+            uint32_t ti = lane & ((1u << FB) - 1);
 #pragma unroll
-        for (int s_ = 0; s_ < WB; s_++) wm[s_] = d.wmap0[s_];
+            for (int s_ = 0; s_ < (int)FL; s_++) ti |= ((lane >> (FB + s_)) & 1) << lm[s_];
+#pragma unroll
+            for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << wmp[s_];
+            return ti;
+        };
+        uint32_t ti = thread_index(d.lmap0, d.wmap0);
+        uint32_t rm[RB];
 #pragma unroll
         for (int b = 0; b < RB; b++) rm[b] = d.rmap0[b];
 
@@ -309,8 +315,13 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                     break;
                 }
                 case V_DIAG:
-                    if (v.form == 1) diag<T2, RB, true>(x, v, cbase, tid, base);
-                    else diag<T2, RB, false>(x, v, cbase, tid, base);
+                    if (v.threaded) {
+                        if (v.form == 1) diag<T2, RB, true, true>(x, v, cbase, tid, base);
+                        else diag<T2, RB, false, true>(x, v, cbase, tid, base);
+                    } else {
+                        if (v.form == 1) diag<T2, RB, true, false>(x, v, cbase, tid, base);
+                        else diag<T2, RB, false, false>(x, v, cbase, tid, base);
+                    }
                     break;
                 case V_SCALE: {
                     if (v.form == 1) {
@@ -331,9 +342,6 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 }
                 case V_XCHG:
                 case V_LDSOP: {
-                    uint32_t ti = lane;
-#pragma unroll
-                    for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << wm[s_];
                     __syncthreads();
 #pragma unroll
                     for (int j = 0; j < R; j++) {
@@ -356,12 +364,8 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                         __syncthreads();
                     } else {
 #pragma unroll
-                        for (int s_ = 0; s_ < WB; s_++) wm[s_] = v.wmap[s_];
-#pragma unroll
                         for (int b = 0; b < RB; b++) rm[b] = v.rmap[b];
-                        ti = lane;
-#pragma unroll
-                        for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << wm[s_];
+                        ti = thread_index(v.lmap, v.wmap);
                     }
 #pragma unroll
                     for (int j = 0; j < R; j++) {
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
 
 template <typename T2>
 VGeom vstage_geom() {
-    return sizeof(T2) == 8 ? VGeom{3, 5} : VGeom{3, 4};
+    return sizeof(T2) == 8 ? VGeom{3, 5, 4} : VGeom{3, 4, 3};
 }
 
 template <typename T2>
@@ -407,12 +411,12 @@ hipError_t launch_vstage(T2 *s, const VStageDesc &d, const VInstr *prog, const T
                          const uint32_t *ints, unsigned grid, hipStream_t st) {
     const size_t lds = vstage_lds_bytes<T2>();
     if constexpr (sizeof(T2) == 8) {
-        auto k = k_vstage<T2, 3, 5>;
+        auto k = k_vstage<T2, 3, 5, 4>;
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         k<<<grid, 64 << 3, lds, st>>>(s, d, prog, coefs, ldsops, ints);
     } else {
-        auto k = k_vstage<T2, 3, 4>;
+        auto k = k_vstage<T2, 3, 4, 3>;
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         k<<<grid, 64 << 3, lds, st>>>(s, d, prog, coefs, ldsops, ints);
