@@ -91,11 +91,83 @@ struct Gen {
     // after the pending diagonal).
     std::vector<M2> pmat;
     std::vector<char> has_pmat;
+    // Pending run of commuting CZ-like pairs (sign = (-1)^(x_p x_q)).
+    std::vector<std::pair<uint32_t, uint32_t>> czrun;
+    std::vector<char> in_run;
 
     Gen(const Stage &s, uint32_t n_, const VGeom &g_, PendingDiag &p, std::vector<C64> &c, std::vector<StageOp> &l,
         std::vector<uint32_t> &ii)
-        : st(s), n(n_), g(g_), pend(p), coefs(c), ldsops(l), ints(ii), tbit(n_, -1), pmat(n_), has_pmat(n_, 0) {
+        : st(s), n(n_), g(g_), pend(p), coefs(c), ldsops(l), ints(ii), tbit(n_, -1), pmat(n_), has_pmat(n_, 0),
+          in_run(n_, 0) {
         for (size_t j = 0; j < s.local.size(); j++) tbit[s.local[j]] = (int)j;
+    }
+
+    // Emits the pending CZ-like run as one V_SIGNS: pairs of two register
+    // bits become a per-register mask; the others go to the int array.
+    void flush_run() {
+        if (czrun.empty()) return;
+        VInstr v = blank(V_SIGNS);
+        v.aux = (uint32_t)ints.size();
+        uint64_t rr = 0;
+        for (auto &pq : czrun) {
+            const uint32_t a = encode(pq.first), b = encode(pq.second);
+            if ((a & VT_KIND) == VT_REG && (b & VT_KIND) == VT_REG) {
+                const uint32_t ra = a & 0xffffu, rb = b & 0xffffu;
+                for (uint32_t j = 0; j < (1u << g.r_bits); j++)
+                    if (((j >> ra) & 1) && ((j >> rb) & 1)) rr ^= uint64_t(1) << j;
+                continue;
+            }
+            ints.push_back(a);
+            ints.push_back(b);
+            v.k++;
+        }
+        v.neg = rr;
+        out.push_back(v);
+        for (auto &pq : czrun) in_run[pq.first] = in_run[pq.second] = 0;
+        czrun.clear();
+    }
+
+    bool touches_run(const Op &op) const {
+        for (uint32_t q : op.qubits())
+            if (in_run[q]) return true;
+        return false;
+    }
+
+    // A diagonal op on at most two qubits whose entries are all +-1: its
+    // sign is (-1)^(c0 + c1 x_a + c2 x_b + c3 x_a x_b). The linear and
+    // constant terms join the pending diagonals; x_a x_b joins the run.
+    bool absorb_sign_diag(const Op &op) {
+        const std::vector<uint32_t> qs = op.qubits();  // targets, then controls
+        if (qs.size() > 2) return false;
+        const size_t kt = op.targets.size();
+        int s[4] = {0, 0, 0, 0};
+        for (uint32_t x = 0; x < (1u << qs.size()); x++) {
+            bool ctrl = true;
+            for (size_t c = 0; c < op.controls.size(); c++)
+                if ((((x >> (kt + c)) & 1) != 0) != op.controls[c].second) ctrl = false;
+            const C64 e = ctrl ? op.m.at(x & ((1u << kt) - 1), x & ((1u << kt) - 1)) : C64(1, 0);
+            if (e == C64(-1, 0)) s[x] = 1;
+            else if (e != C64(1, 0)) return false;
+        }
+        for (uint32_t q : qs) flush_pmat(q);
+        const uint32_t a = qs[0];
+        const int c0 = s[0], c1 = s[1] ^ s[0];
+        auto zflip = [&](uint32_t q) { pend.d1[q] = -pend.d1[q]; };
+        if (c0) {
+            pend.d0[a] = -pend.d0[a];
+            pend.d1[a] = -pend.d1[a];
+        }
+        if (c1) zflip(a);
+        if (qs.size() == 2) {
+            const uint32_t b = qs[1];
+            const int c2 = s[2] ^ s[0], c3 = s[3] ^ s[2] ^ s[1] ^ s[0];
+            if (c2) zflip(b);
+            if (c3) {
+                czrun.push_back({a, b});
+                in_run[a] = in_run[b] = 1;
+            }
+        }
+        return true;
     }
 
     // Where a physical qubit's bit lives now.
@@ -392,9 +464,11 @@ struct Gen {
                 }
                 return;
             }
+            if (absorb_sign_diag(op)) return;
             emit_diag(op);
             return;
         }
+        if (touches_run(op)) flush_run();
         if (op.targets.size() == 1) {
             const uint32_t q = op.targets[0];
             for (auto &c : op.controls) flush_pmat(c.first);
@@ -441,6 +515,7 @@ struct Gen {
             // Pending products are emitted first, in the slots they have now.
             choose_roles(lookahead(done), false, l, w, r);
             for (uint32_t q = 0; q < n; q++) flush_pmat(q);
+            flush_run();
             require(l != roles.lmap || w != roles.wmap || r != roles.rmap, "internal: exchange without progress");
             VInstr v = blank(V_XCHG);
             for (size_t s = 0; s < l.size(); s++) v.lmap[s] = l[s];
@@ -450,6 +525,7 @@ struct Gen {
             out.push_back(v);
         }
         for (uint32_t q = 0; q < n; q++) flush_pmat(q);
+        flush_run();
         if (flush_all) emit_scale();
         for (size_t s = 0; s < roles.lmap.size(); s++) p.desc.lmapF[s] = roles.lmap[s];
         for (size_t s = 0; s < roles.wmap.size(); s++) p.desc.wmapF[s] = roles.wmap[s];

@@ -31,6 +31,7 @@ enum VOp : uint32_t {
     V_XCHG = 3,  // exchange wave bits and register bits through LDS
     V_LDSOP = 4, // any op of up to 4 targets, applied with the tile in LDS
     V_SCALE = 5, // multiply by per-thread x per-register x per-tile factors
+    V_SIGNS = 6, // a run of commuting CZ-like gates: sign = parity of a quadratic form over bits
 };
 
 // Forms of a 1-qubit gate's coefficients c[0..3]:

@@ -19,6 +19,15 @@
 #include "vprog.h"
 #include "vstage.h"
 
+// Constant address space: loads through these pointers are invariant, so
+// uniform ones go through the scalar cache (s_load) and never wait behind
+// the tile's vector loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QVD_CONST __attribute__((address_space(4)))
+#else
+#define QVD_CONST
+#endif
+
 namespace qvd {
 
 namespace {
@@ -62,9 +71,56 @@ __device__ __forceinline__ T2 sel(bool p, T2 a, T2 b) {
     return mk<T2>(p ? a.x : b.x, p ? a.y : b.y);
 }
 
+// Negates a when bit 0 of f is set (sign-bit xor).
+__device__ __forceinline__ float2 flip_sign(float2 a, uint32_t f) {
+    const uint32_t m = f << 31;
+    return make_float2(__uint_as_float(__float_as_uint(a.x) ^ m), __uint_as_float(__float_as_uint(a.y) ^ m));
+}
+__device__ __forceinline__ double2 flip_sign(double2 a, uint32_t f) {
+    const unsigned long long m = (unsigned long long)f << 63;
+    return make_double2(__longlong_as_double(__double_as_longlong(a.x) ^ m),
+                        __longlong_as_double(__double_as_longlong(a.y) ^ m));
+}
+
+// A run of CZ-like gates: amplitude (thread, register j) changes sign when
+// the quadratic form sum x_p x_q over the run's pairs is odd. Pairs of two
+// register bits arrive as the per-register mask v.neg; every other pair is
+// (encoded bit, encoded bit) in the int array.
+template <typename T2, int RB>
+__device__ __forceinline__ void signs(T2 (&x)[1 << RB], const VInstr &v, const QVD_CONST uint32_t *ints,
+                                      uint32_t tid, uint64_t base) {
+    uint32_t al = 0, gam = 0, gm = 0;
+    for (uint32_t i = 0; i < v.k; i++) {
+        const uint32_t a = ints[v.aux + 2 * i], b = ints[v.aux + 2 * i + 1];
+        const uint32_t ka = a & VT_KIND, kb = b & VT_KIND, xa = a & 0xffffu, xb = b & 0xffffu;
+        const uint32_t va = ka == VT_THREAD ? (tid >> xa) & 1 : (uint32_t)(base >> xa) & 1;
+        const uint32_t vb = kb == VT_THREAD ? (tid >> xb) & 1 : (uint32_t)(base >> xb) & 1;
+        if (ka != VT_REG && kb != VT_REG) al ^= va & vb;
+        else if (ka == VT_REG) {
+            if (kb == VT_THREAD) gam ^= vb << xa;
+            else gm ^= vb << xa;
+        } else {
+            if (ka == VT_THREAD) gam ^= va << xb;
+            else gm ^= va << xb;
+        }
+    }
+    const uint32_t mask = gam ^ gm;
+    if (v.k == 0) {
+#pragma unroll
+        for (int j = 0; j < (1 << RB); j++)
+            if ((v.neg >> j) & 1) x[j] = flip_sign(x[j], 1);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        const uint32_t f = al ^ (uint32_t)((v.neg >> j) & 1) ^ __popc((uint32_t)j & mask);
+        x[j] = flip_sign(x[j], f & 1);
+    }
+}
+
 // 1-qubit gate on register bit B.
 template <typename T2, int RB, int B, int FORM, bool CTRL>
-__device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const T2 *__restrict__ c, uint32_t tpred_ok, uint32_t rmask,
+__device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t tpred_ok, uint32_t rmask,
                                     uint32_t rval) {
     const T2 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
     const bool tp = tpred_ok != 0;
@@ -95,7 +151,7 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const T2 *__restrict__ c, 
 }
 
 template <typename T2, int RB, int FORM, bool CTRL>
-__device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const T2 *__restrict__ c, uint32_t tp,
+__device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const QVD_CONST T2 *c, uint32_t tp,
                                          uint32_t rmask, uint32_t rval) {
     switch (slot) {
         case 0: u1r<T2, RB, 0, FORM, CTRL>(x, c, tp, rmask, rval); break;
@@ -110,7 +166,7 @@ __device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const 
 
 // 1-qubit gate on lane bit b: the partner amplitude comes from lane ^ (1 << b).
 template <typename T2, int RB, int FORM, bool CTRL>
-__device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const T2 *__restrict__ c, uint32_t lane,
+__device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONST T2 *c, uint32_t lane,
                                     uint32_t tp, uint32_t rmask, uint32_t rval) {
     const bool hi = (lane >> b) & 1;
     const T2 cs = hi ? c[3] : c[0];  // coefficient of the lane's own amplitude
@@ -167,7 +223,7 @@ __device__ __forceinline__ void diag(T2 (&x)[1 << RB], const VInstr &v, const T2
 
 // In-LDS dense op (the general path): groups striped over the workgroup.
 template <typename T2, int K, int W>
-__device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__restrict__ m, const StageOp &op,
+__device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__restrict__ m, const QVD_CONST StageOp &op,
                                           uint32_t L) {
     constexpr int D = 1 << K;
     uint32_t off[D];
@@ -216,6 +272,10 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     extern __shared__ unsigned char smem_raw[];
     T2 *tile = reinterpret_cast<T2 *>(smem_raw);
     T2 *cst = tile + (1u << L);  // the stage's coefficients (index = coef - cbegin)
+    const QVD_CONST VInstr *cprog = (const QVD_CONST VInstr *)prog;
+    const QVD_CONST T2 *ccoef = (const QVD_CONST T2 *)coefs;
+    const QVD_CONST StageOp *cops = (const QVD_CONST StageOp *)ldsops;
+    const QVD_CONST uint32_t *cints = (const QVD_CONST uint32_t *)ints;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
 
     for (uint32_t i = tid; i < d.ccount; i += W) cst[i] = coefs[d.cbegin + i];
@@ -287,13 +347,13 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         for (int b = 0; b < RB; b++) rm[b] = d.rmap0[b];
 
         for (uint32_t ii = 0; ii < d.num_instr; ii++) {
-            const VInstr &v = prog[d.instr_off + ii];
+            const VInstr v = cprog[d.instr_off + ii];
             if ((base & v.gmask) != v.gval) continue;
             const uint32_t tp = (tid & v.tmask) == v.tval;
             const bool ctrl = v.tmask != 0 || v.rmask != 0;
             switch (v.op) {
                 case V_U1R: {
-                    const T2 *c = coefs + v.coef;
+                    const QVD_CONST T2 *c = ccoef + v.coef;
                     if (!ctrl) {
                         if (v.form == FORM_UNIT_DIAG) u1r_slot<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, 1, 0, 0);
                         else if (v.form == FORM_UNIT_ANTI) u1r_slot<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, 1, 0, 0);
@@ -304,7 +364,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                     break;
                 }
                 case V_U1L: {
-                    const T2 *c = coefs + v.coef;
+                    const QVD_CONST T2 *c = ccoef + v.coef;
                     if (!ctrl) {
                         if (v.form == FORM_UNIT_DIAG) u1l<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, lane, 1, 0, 0);
                         else if (v.form == FORM_UNIT_ANTI) u1l<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, lane, 1, 0, 0);
@@ -323,20 +383,23 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                         else diag<T2, RB, false, false>(x, v, cbase, tid, base);
                     }
                     break;
+                case V_SIGNS:
+                    signs<T2, RB>(x, v, cints, tid, base);
+                    break;
                 case V_SCALE: {
                     if (v.form == 1) {
-                        const T2 f = coefs[v.coef];
+                        const T2 f = ccoef[v.coef];
 #pragma unroll
                         for (int j = 0; j < R; j++) x[j] = cm(f, x[j]);
                     } else {
                         T2 ft = cbase[v.coef + tid];
                         const T2 *g = cbase + v.coef + W + R;
                         for (uint32_t i = 0; i < v.k; i++) {
-                            const uint32_t q = ints[v.aux + i];
+                            const uint32_t q = cints[v.aux + i];
                             ft = cm(g[2 * i + ((base >> q) & 1)], ft);
                         }
 #pragma unroll
-                        for (int j = 0; j < R; j++) x[j] = cm(cm(coefs[v.coef + W + j], ft), x[j]);
+                        for (int j = 0; j < R; j++) x[j] = cm(cm(ccoef[v.coef + W + j], ft), x[j]);
                     }
                     break;
                 }
@@ -353,7 +416,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                     }
                     __syncthreads();
                     if (v.op == V_LDSOP) {
-                        const StageOp &op = ldsops[v.aux];
+                        const QVD_CONST StageOp &op = cops[v.aux];
                         const T2 *m = cbase + op.coef_off;
                         switch (op.k) {
                             case 2: lds_dense<T2, 2, W>(tile, m, op, L); break;
