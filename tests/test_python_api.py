@@ -1,0 +1,124 @@
+"""The qevedo.simulator Python surface, mirroring the state-vector tests of
+the reference's python/tests/test_simulator.py (:77-176). CPU tests cover
+circuit building, OpenQASM and validation; GPU tests run circuits."""
+import cmath
+import math
+import random
+
+import numpy as np
+import pytest
+
+from helpers import oracle_statevector
+from qevedo.simulator import Circuit, run, statevector
+
+
+def test_qasm_round_trip_and_instructions():
+    plain = Circuit(3, 3).h(0).cx(0, 1).rz(0.25, 2).measure(1, 2)
+    again = Circuit.from_qasm(plain.to_qasm())
+    assert again.instructions() == plain.instructions()
+    assert plain.instructions()[2] == ("rz", [2], [0.25])
+    assert plain.instructions()[3] == ("measure", [1], [2.0])
+    with pytest.raises(ValueError, match="unitaries"):
+        Circuit(2).unitary(np.eye(4), [0, 1]).to_qasm()
+
+
+def test_qasm_expressions_and_broadcast():
+    c = Circuit.from_qasm('OPENQASM 2.0;\ninclude "qelib1.inc";\nqreg q[3];\ncreg c[3];\n'
+                          'h q;\nu3(pi/2, -pi/4, pi/4) q[1];\ncu1(pi/8) q[0],q[2];\nmeasure q -> c;\n')
+    ins = c.instructions()
+    assert [i[0] for i in ins[:3]] == ["h", "h", "h"]
+    assert ins[3][0] == "u3" and ins[3][2] == pytest.approx([math.pi / 2, -math.pi / 4, math.pi / 4])
+    assert ins[4] == ("cu1", [0, 2], [math.pi / 8])
+    assert [i[0] for i in ins[5:]] == ["measure"] * 3
+
+
+def test_errors():
+    with pytest.raises(ValueError, match="out of range"):
+        Circuit(2).h(2)
+    with pytest.raises(ValueError, match="same qubit"):
+        Circuit(2).cx(1, 1)
+    with pytest.raises(ValueError, match="line 3"):
+        Circuit.from_qasm('OPENQASM 2.0;\nqreg q[1];\nfoo q[0];')
+    with pytest.raises(ValueError, match="unknown backend"):
+        run(Circuit(1).measure_all(), backend="quantum")
+    with pytest.raises(ValueError, match="unknown gate"):
+        Circuit(1).gate("nope", [0])
+    with pytest.raises(ValueError, match="not unitary"):
+        Circuit(1).unitary([[1, 1], [0, 1]], [0])
+
+
+def probabilities(circuit: Circuit) -> np.ndarray:
+    gates = [(name, q, p) for kind, name, q, p in circuit._ins if kind == "gate"]
+    return np.abs(oracle_statevector(circuit.num_qubits, gates)) ** 2
+
+
+@pytest.mark.gpu
+def test_statevector_matches_known_states():
+    ghz = statevector(Circuit(3).h(0).cx(0, 1).cx(1, 2))
+    r = 1 / math.sqrt(2)
+    assert abs(ghz[0] - r) < 1e-12 and abs(ghz[7] - r) < 1e-12
+    assert sum(abs(a) ** 2 for a in ghz) == pytest.approx(1.0)
+    phased = statevector(Circuit(1).h(0).t(0))
+    assert abs(phased[1] - r * cmath.exp(1j * math.pi / 4)) < 1e-12
+
+
+@pytest.mark.gpu
+def test_samples_the_exact_distribution():
+    rng = random.Random(7)
+    circuit = Circuit(5)
+    for _ in range(40):
+        a, b = rng.sample(range(5), 2)
+        choice = rng.randrange(6)
+        if choice == 0:
+            circuit.h(a)
+        elif choice == 1:
+            circuit.t(a)
+        elif choice == 2:
+            circuit.rx(rng.uniform(0, 6.28), a)
+        elif choice == 3:
+            circuit.cx(a, b)
+        elif choice == 4:
+            circuit.cp(rng.uniform(0, 6.28), a, b)
+        else:
+            circuit.ccx(a, b, rng.choice([q for q in range(5) if q not in (a, b)]))
+    p = probabilities(circuit)
+    circuit.measure_all()
+    shots = 20000
+    result = run(circuit, shots=shots, backend="statevector", seed=5)
+    assert result.backend == "statevector"
+    for i, pi in enumerate(p):
+        observed = result.counts.get(format(i, "05b"), 0) / shots
+        assert abs(observed - pi) <= 5 * math.sqrt(pi * (1 - pi) / shots) + 3 / shots
+
+
+@pytest.mark.gpu
+def test_unitary_from_nested_lists():
+    sx = [[0.5 + 0.5j, 0.5 - 0.5j], [0.5 - 0.5j, 0.5 + 0.5j]]
+    a = statevector(Circuit(1).unitary(sx, [0]))
+    b = statevector(Circuit(1).sx(0))
+    assert all(abs(x - y) < 1e-12 for x, y in zip(a, b))
+
+
+@pytest.mark.gpu
+def test_reproducibility():
+    circuit = Circuit(6)
+    for q in range(6):
+        circuit.h(q).t(q)
+    for q in range(5):
+        circuit.cx(q, q + 1)
+    circuit.measure_all()
+    a = run(circuit, shots=3000, seed=9, backend="statevector", threads=1)
+    b = run(circuit, shots=3000, seed=9, backend="statevector", threads=4)
+    assert a.counts == b.counts
+
+
+@pytest.mark.gpu
+def test_mid_circuit_measurement_reset_and_feedback():
+    # tests/simulator.rs:113-139: x; measure; reset; measure; h; measure
+    c = Circuit(1).x(0).measure(0, 0).reset(0).measure(0, 1).h(0).measure(0, 2)
+    result = run(c, shots=400, seed=3)
+    assert sorted(result.counts) == ["001", "101"]
+    # feedback: measure a |+> qubit, flip it back to |0> when it read 1
+    c = Circuit(1).h(0).measure(0, 0).feedback("X", 0, 0).measure(0, 1)
+    result = run(c, shots=300, seed=4)
+    assert set(result.counts) <= {"00", "01"} and len(result.counts) == 2
