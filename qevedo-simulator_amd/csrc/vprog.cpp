@@ -61,6 +61,15 @@ M2 mul2(const M2 &a, const M2 &b) {
 
 bool has(const std::vector<uint8_t> &v, uint32_t x) { return std::find(v.begin(), v.end(), (uint8_t)x) != v.end(); }
 
+// k when c is i^k to within 1e-14 (the rounding of sin and cos of pi/4 makes
+// RX(pi/2) and RY(pi/2) miss the exact values by an ulp), else -1.
+int quarter_turn(C64 c) {
+    static const C64 rot[4] = {C64(1, 0), C64(0, 1), C64(-1, 0), C64(0, -1)};
+    for (int k = 0; k < 4; k++)
+        if (std::abs(c - rot[k]) < 1e-14) return k;
+    return -1;
+}
+
 // Keeps each bit in its old slot of the same role where it can, so only the
 // bits that move change slots.
 std::vector<uint8_t> stabilize(const std::vector<uint8_t> &want, const std::vector<uint8_t> &old) {
@@ -234,7 +243,15 @@ struct Gen {
         }
         if (nsq(u[0]) >= nsq(u[1])) {
             // [[u0, u1], [u2, u3]] = diag(u0, u3) [[1, u1/u0], [u2/u3, 1]]
-            emit_u1(q, {C64(1, 0), u[1] / u[0], u[2] / u[3], C64(1, 0)}, FORM_UNIT_DIAG, {});
+            const C64 a = u[1] / u[0], b = u[2] / u[3];
+            const int ka = quarter_turn(a), kb = quarter_turn(b);
+            if (ka >= 0 && kb >= 0) {
+                // Off-diagonals in {1, i, -1, -i} (to 1e-14): adds only.
+                emit_u1(q, {C64(1, 0), a, b, C64(1, 0)}, FORM_UNIT_ROT, {});
+                out.back().aux = (uint32_t)(ka | (kb << 2));
+            } else {
+                emit_u1(q, {C64(1, 0), a, b, C64(1, 0)}, FORM_UNIT_DIAG, {});
+            }
             pend.d0[q] = u[0];
             pend.d1[q] = u[3];
         } else {

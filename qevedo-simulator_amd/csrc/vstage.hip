@@ -118,11 +118,26 @@ __device__ __forceinline__ void signs(T2 (&x)[1 << RB], const VInstr &v, const Q
     }
 }
 
+// a + i^K b: adds only.
+template <int K, typename T2>
+__device__ __forceinline__ T2 addrot(const T2 a, const T2 b) {
+    if (K == 0) return mk<T2>(a.x + b.x, a.y + b.y);
+    if (K == 1) return mk<T2>(a.x - b.y, a.y + b.x);
+    if (K == 2) return mk<T2>(a.x - b.x, a.y - b.y);
+    return mk<T2>(a.x + b.y, a.y - b.x);
+}
+
 // 1-qubit gate on register bit B.
-template <typename T2, int RB, int B, int FORM, bool CTRL>
+template <typename T2, int RB, int B, int FORM, bool CTRL, int K1 = 0, int K2 = 0>
 __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t tpred_ok, uint32_t rmask,
                                     uint32_t rval) {
-    const T2 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    T2 c0, c1, c2, c3;
+    if (FORM != FORM_UNIT_ROT) {
+        c0 = c[0];
+        c1 = c[1];
+        c2 = c[2];
+        c3 = c[3];
+    }
     const bool tp = tpred_ok != 0;
 #pragma unroll
     for (int j0 = 0; j0 < (1 << RB); j0++) {
@@ -131,7 +146,10 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uin
         if (CTRL && ((uint32_t)j0 & rmask) != rval) continue;
         const T2 a = x[j0], b = x[j1];
         T2 o0, o1;
-        if (FORM == FORM_UNIT_DIAG) {
+        if (FORM == FORM_UNIT_ROT) {
+            o0 = addrot<K1>(a, b);
+            o1 = addrot<K2>(b, a);
+        } else if (FORM == FORM_UNIT_DIAG) {
             o0 = cfma(c1, b, a);
             o1 = cfma(c2, a, b);
         } else if (FORM == FORM_UNIT_ANTI) {
@@ -150,18 +168,34 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uin
     }
 }
 
-template <typename T2, int RB, int FORM, bool CTRL>
+template <typename T2, int RB, int FORM, bool CTRL, int K1 = 0, int K2 = 0>
 __device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const QVD_CONST T2 *c, uint32_t tp,
                                          uint32_t rmask, uint32_t rval) {
     switch (slot) {
-        case 0: u1r<T2, RB, 0, FORM, CTRL>(x, c, tp, rmask, rval); break;
-        case 1: u1r<T2, RB, 1, FORM, CTRL>(x, c, tp, rmask, rval); break;
-        case 2: u1r<T2, RB, 2, FORM, CTRL>(x, c, tp, rmask, rval); break;
-        case 3: u1r<T2, RB, 3, FORM, CTRL>(x, c, tp, rmask, rval); break;
+        case 0: u1r<T2, RB, 0, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
+        case 1: u1r<T2, RB, 1, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
+        case 2: u1r<T2, RB, 2, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
+        case 3: u1r<T2, RB, 3, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
         default:
-            if constexpr (RB > 4) u1r<T2, RB, (RB > 4 ? 4 : 0), FORM, CTRL>(x, c, tp, rmask, rval);
+            if constexpr (RB > 4) u1r<T2, RB, (RB > 4 ? 4 : 0), FORM, CTRL, K1, K2>(x, c, tp, rmask, rval);
             break;
     }
+}
+
+// FORM_UNIT_ROT on a register bit: k = k1 | (k2 << 2).
+template <typename T2, int RB>
+__device__ __forceinline__ void u1r_rot(T2 (&x)[1 << RB], uint32_t slot, uint32_t k) {
+    const QVD_CONST T2 *none = nullptr;
+#define QVD_ROT(K1, K2) \
+    case (K1 | (K2 << 2)): u1r_slot<T2, RB, FORM_UNIT_ROT, false, K1, K2>(x, slot, none, 1, 0, 0); break;
+    switch (k) {
+        QVD_ROT(0, 0) QVD_ROT(1, 0) QVD_ROT(2, 0) QVD_ROT(3, 0)
+        QVD_ROT(0, 1) QVD_ROT(1, 1) QVD_ROT(2, 1) QVD_ROT(3, 1)
+        QVD_ROT(0, 2) QVD_ROT(1, 2) QVD_ROT(2, 2) QVD_ROT(3, 2)
+        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3) QVD_ROT(3, 3)
+        default: break;
+    }
+#undef QVD_ROT
 }
 
 // 1-qubit gate on lane bit b: the partner amplitude comes from lane ^ (1 << b).
@@ -355,7 +389,8 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 case V_U1R: {
                     const QVD_CONST T2 *c = ccoef + v.coef;
                     if (!ctrl) {
-                        if (v.form == FORM_UNIT_DIAG) u1r_slot<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, 1, 0, 0);
+                        if (v.form == FORM_UNIT_ROT) u1r_rot<T2, RB>(x, v.slot, v.aux);
+                        else if (v.form == FORM_UNIT_DIAG) u1r_slot<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, 1, 0, 0);
                         else if (v.form == FORM_UNIT_ANTI) u1r_slot<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, 1, 0, 0);
                         else u1r_slot<T2, RB, FORM_FULL, false>(x, v.slot, c, 1, 0, 0);
                     } else {
@@ -366,7 +401,9 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 case V_U1L: {
                     const QVD_CONST T2 *c = ccoef + v.coef;
                     if (!ctrl) {
-                        if (v.form == FORM_UNIT_DIAG) u1l<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, lane, 1, 0, 0);
+                        // the rotation differs between the two lanes of a pair: multiply-add form
+                        if (v.form == FORM_UNIT_DIAG || v.form == FORM_UNIT_ROT)
+                            u1l<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, lane, 1, 0, 0);
                         else if (v.form == FORM_UNIT_ANTI) u1l<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, lane, 1, 0, 0);
                         else u1l<T2, RB, FORM_FULL, false>(x, v.slot, c, lane, 1, 0, 0);
                     } else {
