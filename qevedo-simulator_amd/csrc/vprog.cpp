@@ -570,6 +570,10 @@ VProgram build_vprogram(const Stage &stage, uint32_t n, const VGeom &g, PendingD
         if (gen.tbit[q] < 0) d.glob_pos[d.nglob++] = q;
     d.num_tiles = uint64_t(1) << d.nglob;
     d.num_instr = (uint32_t)p.instr.size();
+    for (VInstr &v : p.instr) {
+        if (v.tmask != 0 || v.rmask != 0) v.op |= VF_CTRL;
+        if (v.gmask != 0) v.op |= VF_GCTRL;
+    }
     return p;
 }
 

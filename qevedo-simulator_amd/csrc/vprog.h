@@ -34,6 +34,12 @@ enum VOp : uint32_t {
     V_SIGNS = 6, // a run of commuting CZ-like gates: sign = parity of a quadratic form over bits
 };
 
+// VInstr::op holds the VOp in its low byte and these flags above it, so the
+// kernel reads one word to dispatch and loads other fields only where used.
+constexpr uint32_t VOP_MASK = 0xffu;
+constexpr uint32_t VF_CTRL = 1u << 8;   // controls on thread or register bits (tmask / rmask nonzero)
+constexpr uint32_t VF_GCTRL = 1u << 9;  // controls on non-tile qubits (gmask nonzero)
+
 // Forms of a 1-qubit gate's coefficients c[0..3]:
 //   FORM_FULL      [[c0, c1], [c2, c3]]
 //   FORM_UNIT_DIAG [[1, c1], [c2, 1]]   (4 fused multiply-adds per amplitude)
@@ -52,7 +58,7 @@ constexpr uint32_t VMAX_L_FLEX = 4;  // flexible lane bits
 constexpr uint32_t VMAX_REGS = 32;
 
 struct VInstr {
-    uint32_t op;
+    uint32_t op;    // VOp | VF_* flags
     uint32_t form;
     uint32_t slot;  // U1R: register bit; U1L: lane bit
     uint32_t k;     // DIAG: number of targets; SCALE: number of non-tile factors

@@ -59,11 +59,88 @@ __device__ __forceinline__ T2 cdot(const T2 w0, const T2 x0, const T2 w1, const 
     return cfma(w1, x1, cm(w0, x0));
 }
 
-__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
-    return make_float2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+// Value of v in lane ^ (1 << B): DPP quad permutes for bits 0 and 1, a
+// half-row mirror followed by a quad reversal for bit 2, a row rotation by 8
+// for bit 3, and the row-swap instructions for bits 4 and 5. All 64 lanes are
+// active wherever this is called.
+template <int B>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane) {
+    if constexpr (B == 0) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (B == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (B == 2) {
+        const int m = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror: 7 - i
+        return (uint32_t)__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false);      // quad_perm [3,2,1,0]
+    } else {
+        static_assert(B == 3, "lane bits 4 and 5 use row_swap");
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    }
 }
-__device__ __forceinline__ double2 shfl_xor2(double2 v, int m) {
-    return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+
+// Row swaps of gfx950 (B = 4: 16-lane rows, B = 5: 32-lane halves): the lanes
+// with bit B set of a trade places with the lanes with bit B clear of b.
+// Afterwards a holds (a's amplitude with bit B clear, b's amplitude with bit B
+// clear) in its low / high lanes and b holds the two with bit B set, so a pair
+// of registers carries two complete amplitude pairs; the same call undoes it.
+template <int B>
+__device__ __forceinline__ void row_swap(uint32_t &a, uint32_t &b) {
+    if constexpr (B == 4) {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    }
+}
+template <int B>
+__device__ __forceinline__ void row_swap2(float2 &a, float2 &b) {
+    uint32_t ax = __float_as_uint(a.x), ay = __float_as_uint(a.y), bx = __float_as_uint(b.x),
+             by = __float_as_uint(b.y);
+    row_swap<B>(ax, bx);
+    row_swap<B>(ay, by);
+    a = make_float2(__uint_as_float(ax), __uint_as_float(ay));
+    b = make_float2(__uint_as_float(bx), __uint_as_float(by));
+}
+__device__ __forceinline__ void split64(double v, uint32_t &lo, uint32_t &hi) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    lo = (uint32_t)u;
+    hi = (uint32_t)(u >> 32);
+}
+__device__ __forceinline__ double join64(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int B>
+__device__ __forceinline__ void row_swap2(double2 &a, double2 &b) {
+    uint32_t a0, a1, a2, a3, b0, b1, b2, b3;
+    split64(a.x, a0, a1);
+    split64(a.y, a2, a3);
+    split64(b.x, b0, b1);
+    split64(b.y, b2, b3);
+    row_swap<B>(a0, b0);
+    row_swap<B>(a1, b1);
+    row_swap<B>(a2, b2);
+    row_swap<B>(a3, b3);
+    a = make_double2(join64(a0, a1), join64(a2, a3));
+    b = make_double2(join64(b0, b1), join64(b2, b3));
+}
+
+template <int B>
+__device__ __forceinline__ float2 lane_xor2(float2 v, uint32_t lane) {
+    return make_float2(__uint_as_float(lane_xor<B>(__float_as_uint(v.x), lane)),
+                       __uint_as_float(lane_xor<B>(__float_as_uint(v.y), lane)));
+}
+template <int B>
+__device__ __forceinline__ double lane_xor_f64(double v, uint32_t lane) {
+    uint32_t lo, hi;
+    split64(v, lo, hi);
+    return join64(lane_xor<B>(lo, lane), lane_xor<B>(hi, lane));
+}
+template <int B>
+__device__ __forceinline__ double2 lane_xor2(double2 v, uint32_t lane) {
+    return make_double2(lane_xor_f64<B>(v.x, lane), lane_xor_f64<B>(v.y, lane));
 }
 
 template <typename T2>
@@ -87,7 +164,7 @@ __device__ __forceinline__ double2 flip_sign(double2 a, uint32_t f) {
 // register bits arrive as the per-register mask v.neg; every other pair is
 // (encoded bit, encoded bit) in the int array.
 template <typename T2, int RB>
-__device__ __forceinline__ void signs(T2 (&x)[1 << RB], const VInstr &v, const QVD_CONST uint32_t *ints,
+__device__ __forceinline__ void signs(T2 (&x)[1 << RB], const QVD_CONST VInstr &v, const QVD_CONST uint32_t *ints,
                                       uint32_t tid, uint64_t base) {
     uint32_t al = 0, gam = 0, gm = 0;
     for (uint32_t i = 0; i < v.k; i++) {
@@ -198,19 +275,18 @@ __device__ __forceinline__ void u1r_rot(T2 (&x)[1 << RB], uint32_t slot, uint32_
 #undef QVD_ROT
 }
 
-// 1-qubit gate on lane bit b: the partner amplitude comes from lane ^ (1 << b).
-template <typename T2, int RB, int FORM, bool CTRL>
-__device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONST T2 *c, uint32_t lane,
-                                    uint32_t tp, uint32_t rmask, uint32_t rval) {
-    const bool hi = (lane >> b) & 1;
+// 1-qubit gate on lane bit B: the partner amplitude comes from lane ^ (1 << B).
+template <typename T2, int RB, int FORM, bool CTRL, int B>
+__device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t lane, uint32_t tp,
+                                        uint32_t rmask, uint32_t rval) {
+    const bool hi = (lane >> B) & 1;
     const T2 cs = hi ? c[3] : c[0];  // coefficient of the lane's own amplitude
     const T2 cp = hi ? c[2] : c[1];  // coefficient of the partner's
-    const int m = 1 << b;
 #pragma unroll
     for (int j = 0; j < (1 << RB); j++) {
         if (CTRL && ((uint32_t)j & rmask) != rval) continue;
         const T2 a = x[j];
-        const T2 p = shfl_xor2(a, m);
+        const T2 p = lane_xor2<B>(a, lane);
         T2 o;
         if (FORM == FORM_UNIT_DIAG) o = cfma(cp, p, a);
         else if (FORM == FORM_UNIT_ANTI) o = cfma(cs, a, p);
@@ -220,12 +296,88 @@ __device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONS
     }
 }
 
+// 1-qubit gate on lane bit B = 4 or 5: registers 2i and 2i + 1 are row-swapped
+// so that each lane holds a whole amplitude pair (of register 2i in the lanes
+// with bit B clear, of 2i + 1 in the others), the gate runs with uniform
+// coefficients as on a register bit, and a second swap puts the results back.
+template <typename T2, int RB, int FORM, bool CTRL, int B, int K1 = 0, int K2 = 0>
+__device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t lane, uint32_t tp,
+                                         uint32_t rmask, uint32_t rval) {
+    T2 c0, c1, c2, c3;
+    if (FORM != FORM_UNIT_ROT) {
+        c0 = c[0];
+        c1 = c[1];
+        c2 = c[2];
+        c3 = c[3];
+    }
+    const bool hi = (lane >> B) & 1;
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j += 2) {
+        const uint32_t reg = hi ? (uint32_t)j + 1 : (uint32_t)j;
+        if (CTRL && (rmask & 1) == 0 && (((uint32_t)j & rmask) != rval)) continue;  // neither register selected
+        T2 a = x[j], b = x[j + 1];
+        row_swap2<B>(a, b);
+        T2 o0, o1;
+        if (FORM == FORM_UNIT_ROT) {
+            o0 = addrot<K1>(a, b);
+            o1 = addrot<K2>(b, a);
+        } else if (FORM == FORM_UNIT_DIAG) {
+            o0 = cfma(c1, b, a);
+            o1 = cfma(c2, a, b);
+        } else if (FORM == FORM_UNIT_ANTI) {
+            o0 = cfma(c0, a, b);
+            o1 = cfma(c3, b, a);
+        } else {
+            o0 = cdot(c0, a, c1, b);
+            o1 = cdot(c2, a, c3, b);
+        }
+        if (CTRL) {
+            const bool on = tp != 0 && (reg & rmask) == rval;
+            o0 = sel(on, o0, a);
+            o1 = sel(on, o1, b);
+        }
+        row_swap2<B>(o0, o1);
+        x[j] = o0;
+        x[j + 1] = o1;
+    }
+}
+
+template <typename T2, int RB, int FORM, bool CTRL>
+__device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONST T2 *c, uint32_t lane,
+                                    uint32_t tp, uint32_t rmask, uint32_t rval) {
+    switch (b) {
+        case 0: u1l_bit<T2, RB, FORM, CTRL, 0>(x, c, lane, tp, rmask, rval); break;
+        case 1: u1l_bit<T2, RB, FORM, CTRL, 1>(x, c, lane, tp, rmask, rval); break;
+        case 2: u1l_bit<T2, RB, FORM, CTRL, 2>(x, c, lane, tp, rmask, rval); break;
+        case 3: u1l_bit<T2, RB, FORM, CTRL, 3>(x, c, lane, tp, rmask, rval); break;
+        case 4: u1l_pair<T2, RB, FORM, CTRL, 4>(x, c, lane, tp, rmask, rval); break;
+        default: u1l_pair<T2, RB, FORM, CTRL, 5>(x, c, lane, tp, rmask, rval); break;
+    }
+}
+
+// FORM_UNIT_ROT on lane bit 4 or 5 (k = k1 | (k2 << 2)); other lane bits take
+// the multiply-add form, since the two lanes of a pair need different rotations.
+template <typename T2, int RB, int B>
+__device__ __forceinline__ void u1l_rot_pair(T2 (&x)[1 << RB], uint32_t lane, uint32_t k) {
+    const QVD_CONST T2 *none = nullptr;
+#define QVD_ROT(K1, K2) \
+    case (K1 | (K2 << 2)): u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K1, K2>(x, none, lane, 1, 0, 0); break;
+    switch (k) {
+        QVD_ROT(0, 0) QVD_ROT(1, 0) QVD_ROT(2, 0) QVD_ROT(3, 0)
+        QVD_ROT(0, 1) QVD_ROT(1, 1) QVD_ROT(2, 1) QVD_ROT(3, 1)
+        QVD_ROT(0, 2) QVD_ROT(1, 2) QVD_ROT(2, 2) QVD_ROT(3, 2)
+        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3) QVD_ROT(3, 3)
+        default: break;
+    }
+#undef QVD_ROT
+}
+
 // A diagonal op: entry index = (thread-index part, per lane) | (non-tile
 // part, per tile) | rtab[j] (register part, precomputed per register).
 // THREADED = false when no target or control is a thread-index bit: then
 // every decision is uniform and sign flips are plain branches.
 template <typename T2, int RB, bool SIGN, bool THREADED>
-__device__ __forceinline__ void diag(T2 (&x)[1 << RB], const VInstr &v, const T2 *__restrict__ cst, uint32_t tid,
+__device__ __forceinline__ void diag(T2 (&x)[1 << RB], const QVD_CONST VInstr &v, const T2 *__restrict__ cst, uint32_t tid,
                                      uint64_t base) {
     uint32_t it = 0, ig = 0;
 #pragma unroll
@@ -381,11 +533,11 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         for (int b = 0; b < RB; b++) rm[b] = d.rmap0[b];
 
         for (uint32_t ii = 0; ii < d.num_instr; ii++) {
-            const VInstr v = cprog[d.instr_off + ii];
-            if ((base & v.gmask) != v.gval) continue;
-            const uint32_t tp = (tid & v.tmask) == v.tval;
-            const bool ctrl = v.tmask != 0 || v.rmask != 0;
-            switch (v.op) {
+            const QVD_CONST VInstr &v = cprog[d.instr_off + ii];
+            const uint32_t w = v.op;
+            if ((w & VF_GCTRL) && (base & v.gmask) != v.gval) continue;
+            const bool ctrl = (w & VF_CTRL) != 0;
+            switch (w & VOP_MASK) {
                 case V_U1R: {
                     const QVD_CONST T2 *c = ccoef + v.coef;
                     if (!ctrl) {
@@ -394,6 +546,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                         else if (v.form == FORM_UNIT_ANTI) u1r_slot<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, 1, 0, 0);
                         else u1r_slot<T2, RB, FORM_FULL, false>(x, v.slot, c, 1, 0, 0);
                     } else {
+                        const uint32_t tp = (tid & v.tmask) == v.tval;
                         u1r_slot<T2, RB, FORM_FULL, true>(x, v.slot, c, tp, v.rmask, v.rval);
                     }
                     break;
@@ -401,12 +554,15 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 case V_U1L: {
                     const QVD_CONST T2 *c = ccoef + v.coef;
                     if (!ctrl) {
-                        // the rotation differs between the two lanes of a pair: multiply-add form
-                        if (v.form == FORM_UNIT_DIAG || v.form == FORM_UNIT_ROT)
+                        if (v.form == FORM_UNIT_ROT && v.slot >= 4) {
+                            if (v.slot == 4) u1l_rot_pair<T2, RB, 4>(x, lane, v.aux);
+                            else u1l_rot_pair<T2, RB, 5>(x, lane, v.aux);
+                        } else if (v.form == FORM_UNIT_DIAG || v.form == FORM_UNIT_ROT)
                             u1l<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, lane, 1, 0, 0);
                         else if (v.form == FORM_UNIT_ANTI) u1l<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, lane, 1, 0, 0);
                         else u1l<T2, RB, FORM_FULL, false>(x, v.slot, c, lane, 1, 0, 0);
                     } else {
+                        const uint32_t tp = (tid & v.tmask) == v.tval;
                         u1l<T2, RB, FORM_FULL, true>(x, v.slot, c, lane, tp, v.rmask, v.rval);
                     }
                     break;
@@ -452,7 +608,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                         tile[ti | o] = x[j];
                     }
                     __syncthreads();
-                    if (v.op == V_LDSOP) {
+                    if ((w & VOP_MASK) == V_LDSOP) {
                         const QVD_CONST StageOp &op = cops[v.aux];
                         const T2 *m = cbase + op.coef_off;
                         switch (op.k) {
