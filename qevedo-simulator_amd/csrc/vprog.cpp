@@ -100,8 +100,13 @@ struct Gen {
     // after the pending diagonal).
     std::vector<M2> pmat;
     std::vector<char> has_pmat;
-    // Pending run of commuting CZ-like pairs (sign = (-1)^(x_p x_q)).
-    std::vector<std::pair<uint32_t, uint32_t>> czrun;
+    // Pending run of commuting pair phases: the amplitude gains w when both
+    // qubits p and q are 1. A run of w = -1 only is a sign run.
+    struct PairPhase {
+        uint32_t p, q;
+        C64 w;
+    };
+    std::vector<PairPhase> prun;
     std::vector<char> in_run;
 
     Gen(const Stage &s, uint32_t n_, const VGeom &g_, PendingDiag &p, std::vector<C64> &c, std::vector<StageOp> &l,
@@ -111,29 +116,54 @@ struct Gen {
         for (size_t j = 0; j < s.local.size(); j++) tbit[s.local[j]] = (int)j;
     }
 
-    // Emits the pending CZ-like run as one V_SIGNS: pairs of two register
-    // bits become a per-register mask; the others go to the int array.
+    // Emits the pending run: as one V_SIGNS when every pair phase is -1
+    // (pairs of two register bits become a per-register mask, the others go
+    // to the int array as encoded bit pairs), else as one V_PHASES (pairs of
+    // two register bits become a per-register factor table; the others go to
+    // the int array as (encoded bit, encoded bit, coefficient index), a
+    // register bit first when there is one).
     void flush_run() {
-        if (czrun.empty()) return;
-        VInstr v = blank(V_SIGNS);
+        if (prun.empty()) return;
+        bool sign = true;
+        for (auto &pp : prun)
+            if (pp.w != C64(-1, 0)) sign = false;
+        const uint32_t R = 1u << g.r_bits;
+        VInstr v = blank(sign ? V_SIGNS : V_PHASES);
         v.aux = (uint32_t)ints.size();
         uint64_t rr = 0;
-        for (auto &pq : czrun) {
-            const uint32_t a = encode(pq.first), b = encode(pq.second);
+        std::vector<C64> rtable(R, C64(1, 0));
+        bool has_rr = false;
+        for (auto &pp : prun) {
+            uint32_t a = encode(pp.p), b = encode(pp.q);
             if ((a & VT_KIND) == VT_REG && (b & VT_KIND) == VT_REG) {
                 const uint32_t ra = a & 0xffffu, rb = b & 0xffffu;
-                for (uint32_t j = 0; j < (1u << g.r_bits); j++)
-                    if (((j >> ra) & 1) && ((j >> rb) & 1)) rr ^= uint64_t(1) << j;
+                for (uint32_t j = 0; j < R; j++)
+                    if (((j >> ra) & 1) && ((j >> rb) & 1)) {
+                        rr ^= uint64_t(1) << j;
+                        rtable[j] *= pp.w;
+                    }
+                has_rr = true;
                 continue;
             }
+            if ((b & VT_KIND) == VT_REG) std::swap(a, b);
             ints.push_back(a);
             ints.push_back(b);
+            if (!sign) {
+                ints.push_back((uint32_t)coefs.size());
+                coefs.push_back(pp.w);
+            }
             v.k++;
         }
-        v.neg = rr;
+        if (sign) {
+            v.neg = rr;
+        } else if (has_rr) {
+            v.form = 1;
+            v.coef = (uint32_t)coefs.size();
+            for (const C64 &c : rtable) coefs.push_back(c);
+        }
         out.push_back(v);
-        for (auto &pq : czrun) in_run[pq.first] = in_run[pq.second] = 0;
-        czrun.clear();
+        for (auto &pp : prun) in_run[pp.p] = in_run[pp.q] = 0;
+        prun.clear();
     }
 
     bool touches_run(const Op &op) const {
@@ -142,41 +172,61 @@ struct Gen {
         return false;
     }
 
-    // A diagonal op on at most two qubits whose entries are all +-1: its
-    // sign is (-1)^(c0 + c1 x_a + c2 x_b + c3 x_a x_b). The linear and
-    // constant terms join the pending diagonals; x_a x_b joins the run.
-    bool absorb_sign_diag(const Op &op) {
+    // A diagonal op on at most two qubits (targets and controls together):
+    // its entry for bits (x_a, x_b) factors as
+    //   d00 * (d10 / d00)^x_a * (d01 / d00)^x_b * w^(x_a x_b),
+    //   w = d11 d00 / (d10 d01),
+    // so the constant and linear factors join the pending diagonals and w
+    // joins the run of pair phases. Entries that are all +-1 keep exact signs.
+    bool absorb_pair_diag(const Op &op) {
         const std::vector<uint32_t> qs = op.qubits();  // targets, then controls
         if (qs.size() > 2) return false;
         const size_t kt = op.targets.size();
-        int s[4] = {0, 0, 0, 0};
+        C64 e[4] = {C64(1, 0), C64(1, 0), C64(1, 0), C64(1, 0)};
+        bool signs = true;
         for (uint32_t x = 0; x < (1u << qs.size()); x++) {
             bool ctrl = true;
             for (size_t c = 0; c < op.controls.size(); c++)
                 if ((((x >> (kt + c)) & 1) != 0) != op.controls[c].second) ctrl = false;
-            const C64 e = ctrl ? op.m.at(x & ((1u << kt) - 1), x & ((1u << kt) - 1)) : C64(1, 0);
-            if (e == C64(-1, 0)) s[x] = 1;
-            else if (e != C64(1, 0)) return false;
+            e[x] = ctrl ? op.m.at(x & ((1u << kt) - 1), x & ((1u << kt) - 1)) : C64(1, 0);
+            if (e[x] != C64(1, 0) && e[x] != C64(-1, 0)) signs = false;
+            if (e[x] == C64(0, 0)) return false;  // not unitary: leave it to V_DIAG
         }
         for (uint32_t q : qs) flush_pmat(q);
         const uint32_t a = qs[0];
-        const int c0 = s[0], c1 = s[1] ^ s[0];
-        auto zflip = [&](uint32_t q) { pend.d1[q] = -pend.d1[q]; };
-        if (c0) {
-            pend.d0[a] = -pend.d0[a];
-            pend.d1[a] = -pend.d1[a];
+        if (signs) {
+            // (-1)^(c0 + c1 x_a + c2 x_b + c3 x_a x_b), kept exact
+            int s[4];
+            for (int x = 0; x < 4; x++) s[x] = e[x] == C64(-1, 0);
+            const int c0 = s[0], c1 = s[1] ^ s[0];
+            auto zflip = [&](uint32_t q) { pend.d1[q] = -pend.d1[q]; };
+            if (c0) {
+                pend.d0[a] = -pend.d0[a];
+                pend.d1[a] = -pend.d1[a];
+            }
+            if (c1) zflip(a);
+            if (qs.size() == 2) {
+                const uint32_t b = qs[1];
+                const int c2 = s[2] ^ s[0], c3 = s[3] ^ s[2] ^ s[1] ^ s[0];
+                if (c2) zflip(b);
+                if (c3) add_pair(a, b, C64(-1, 0));
+            }
+            return true;
         }
-        if (c1) zflip(a);
+        pend.d0[a] *= e[0];
+        pend.d1[a] *= e[1];
         if (qs.size() == 2) {
             const uint32_t b = qs[1];
-            const int c2 = s[2] ^ s[0], c3 = s[3] ^ s[2] ^ s[1] ^ s[0];
-            if (c2) zflip(b);
-            if (c3) {
-                czrun.push_back({a, b});
-                in_run[a] = in_run[b] = 1;
-            }
+            pend.d1[b] *= e[2] / e[0];
+            const C64 w = e[3] * e[0] / (e[1] * e[2]);
+            if (w != C64(1, 0)) add_pair(a, b, w);
         }
         return true;
+    }
+
+    void add_pair(uint32_t a, uint32_t b, C64 w) {
+        prun.push_back({a, b, w});
+        in_run[a] = in_run[b] = 1;
     }
 
     // Where a physical qubit's bit lives now.
@@ -481,7 +531,7 @@ struct Gen {
                 }
                 return;
             }
-            if (absorb_sign_diag(op)) return;
+            if (absorb_pair_diag(op)) return;
             emit_diag(op);
             return;
         }

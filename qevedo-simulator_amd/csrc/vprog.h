@@ -32,6 +32,7 @@ enum VOp : uint32_t {
     V_LDSOP = 4, // any op of up to 4 targets, applied with the tile in LDS
     V_SCALE = 5, // multiply by per-thread x per-register x per-tile factors
     V_SIGNS = 6, // a run of commuting CZ-like gates: sign = parity of a quadratic form over bits
+    V_PHASES = 7, // a run of commuting pair phases: factor = product of w over pairs with both bits 1
 };
 
 // VInstr::op holds the VOp in its low byte and these flags above it, so the

@@ -263,16 +263,18 @@ __device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const 
 template <typename T2, int RB>
 __device__ __forceinline__ void u1r_rot(T2 (&x)[1 << RB], uint32_t slot, uint32_t k) {
     const QVD_CONST T2 *none = nullptr;
+#define QVD_ROT_BODY(K1, K2) u1r_slot<T2, RB, FORM_UNIT_ROT, false, K1, K2>(x, slot, none, 1, 0, 0); break;
 #define QVD_ROT(K1, K2) \
-    case (K1 | (K2 << 2)): u1r_slot<T2, RB, FORM_UNIT_ROT, false, K1, K2>(x, slot, none, 1, 0, 0); break;
-    switch (k) {
+    case (K1 | (K2 << 2)): QVD_ROT_BODY(K1, K2)
+    switch (k) {  // k < 16: the last combination takes the default arm
         QVD_ROT(0, 0) QVD_ROT(1, 0) QVD_ROT(2, 0) QVD_ROT(3, 0)
         QVD_ROT(0, 1) QVD_ROT(1, 1) QVD_ROT(2, 1) QVD_ROT(3, 1)
         QVD_ROT(0, 2) QVD_ROT(1, 2) QVD_ROT(2, 2) QVD_ROT(3, 2)
-        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3) QVD_ROT(3, 3)
-        default: break;
+        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3)
+        default: QVD_ROT_BODY(3, 3)
     }
 #undef QVD_ROT
+#undef QVD_ROT_BODY
 }
 
 // 1-qubit gate on lane bit B: the partner amplitude comes from lane ^ (1 << B).
@@ -360,16 +362,64 @@ __device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONS
 template <typename T2, int RB, int B>
 __device__ __forceinline__ void u1l_rot_pair(T2 (&x)[1 << RB], uint32_t lane, uint32_t k) {
     const QVD_CONST T2 *none = nullptr;
+#define QVD_ROT_BODY(K1, K2) u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K1, K2>(x, none, lane, 1, 0, 0); break;
 #define QVD_ROT(K1, K2) \
-    case (K1 | (K2 << 2)): u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K1, K2>(x, none, lane, 1, 0, 0); break;
-    switch (k) {
+    case (K1 | (K2 << 2)): QVD_ROT_BODY(K1, K2)
+    switch (k) {  // k < 16: the last combination takes the default arm
         QVD_ROT(0, 0) QVD_ROT(1, 0) QVD_ROT(2, 0) QVD_ROT(3, 0)
         QVD_ROT(0, 1) QVD_ROT(1, 1) QVD_ROT(2, 1) QVD_ROT(3, 1)
         QVD_ROT(0, 2) QVD_ROT(1, 2) QVD_ROT(2, 2) QVD_ROT(3, 2)
-        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3) QVD_ROT(3, 3)
-        default: break;
+        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3)
+        default: QVD_ROT_BODY(3, 3)
     }
 #undef QVD_ROT
+#undef QVD_ROT_BODY
+}
+
+// A run of pair phases: amplitude (thread, register j) gains w for every
+// pair whose two bits are 1. Pairs with a register bit r scale the per-thread
+// factor P[r] (w where the other bit is 1); pairs without one scale the
+// per-thread factor T; pairs of two register bits arrive as the table R[j]
+// (form 1). Register j then takes T * R[j] * prod over set bits b of j of
+// P[b], built as F[j] = F[j with its lowest set bit cleared] * P[that bit].
+template <typename T2, int RB>
+__device__ __forceinline__ void phases(T2 (&x)[1 << RB], const QVD_CONST VInstr &v, const QVD_CONST T2 *coef,
+                                       const QVD_CONST uint32_t *ints, uint32_t tid, uint64_t base) {
+    const T2 one = mk<T2>(1, 0);
+    T2 t = one;
+    T2 pr[RB];
+#pragma unroll
+    for (int b = 0; b < RB; b++) pr[b] = one;
+    const uint32_t k = v.k, off = v.aux;
+    for (uint32_t i = 0; i < k; i++) {
+        const uint32_t a = ints[off + 3 * i], b = ints[off + 3 * i + 1];
+        const T2 w = coef[ints[off + 3 * i + 2]];
+        const uint32_t xb = b & 0xffffu;
+        const bool vb = (b & VT_KIND) == VT_THREAD ? ((tid >> xb) & 1) != 0 : ((base >> xb) & 1) != 0;
+        const uint32_t xa = a & 0xffffu;
+        if ((a & VT_KIND) == VT_REG) {
+            const T2 f = sel(vb, w, one);
+#pragma unroll
+            for (int r = 0; r < RB; r++)
+                if ((uint32_t)r == xa) pr[r] = cm(f, pr[r]);
+        } else {
+            const bool va = (a & VT_KIND) == VT_THREAD ? ((tid >> xa) & 1) != 0 : ((base >> xa) & 1) != 0;
+            t = cm(sel(va && vb, w, one), t);
+        }
+    }
+    // f[j] is live only until its last child; the barriers keep the
+    // scheduler from hoisting every product ahead of its use.
+    const bool table = v.form == 1;
+    const QVD_CONST T2 *rt = coef + v.coef;
+    T2 f[1 << RB];
+    f[0] = t;
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        if (j) f[j] = cm(pr[__builtin_ctz(j)], f[j & (j - 1)]);
+        const T2 g = table ? cm(rt[j], f[j]) : f[j];
+        x[j] = cm(g, x[j]);
+        if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
 // A diagonal op: entry index = (thread-index part, per lane) | (non-tile
@@ -578,6 +628,9 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                     break;
                 case V_SIGNS:
                     signs<T2, RB>(x, v, cints, tid, base);
+                    break;
+                case V_PHASES:
+                    phases<T2, RB>(x, v, ccoef, cints, tid, base);
                     break;
                 case V_SCALE: {
                     if (v.form == 1) {
