@@ -41,6 +41,48 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def run_batch(count: int) -> dict:
+    """Batching workload: `count` hardware-efficient-ansatz circuits (6 layers
+    of RY, RZ on every qubit plus a CZ ladder; qubit counts uniform in 12..22,
+    angles uniform in [0, 2 pi)) in complex64, <Z_i> and <Z_i Z_i+1> of each:
+    one graph-captured batch against the same circuits run one by one."""
+    import numpy as np
+    from qevedo.simulator import Batch, StateVector, library, lower_gates
+    from qevedo.simulator import _native
+
+    rng = np.random.Generator(np.random.PCG64(2024))
+    sizes = [int(x) for x in rng.integers(12, 23, size=count)]
+    circuits = [(n, _native.OpArray(lower_gates(library.hardware_efficient_ansatz(n, 6, 1000 + i))))
+                for i, n in enumerate(sizes)]
+    t = time.perf_counter()
+    b = Batch(circuits, "single")
+    build_s = time.perf_counter() - t
+    b.run()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = b.run()
+    batch_s = (time.perf_counter() - t) / reps
+    stats = b.stats
+    b.close()
+    # the same circuits one by one: plan, run and expectation values per state
+    t = time.perf_counter()
+    worst = 0.0
+    for i, (n, ops) in enumerate(circuits):
+        sv = StateVector(n, "single")
+        sv.apply_ops(ops)
+        z, zz = sv.expect_z()
+        sv.close()
+        worst = max(worst, float(np.max(np.abs(z - out[i][0]))), float(np.max(np.abs(zz - out[i][1]))) if n > 1 else 0.0)
+    single_s = time.perf_counter() - t
+    log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run ({stats['passes']} rounds, "
+        f"{stats['stages']} stage descriptors; build {build_s:.2f} s); one by one {single_s:.3f} s; "
+        f"max |diff| {worst:.2e}")
+    return {"circuits": count, "qubits": "12..22", "seconds": batch_s, "one_by_one_seconds": single_s,
+            "speedup": single_s / batch_s, "rounds": stats["passes"], "build_seconds": build_s,
+            "max_abs_diff_vs_one_by_one": worst}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +97,8 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true", help="skip the complex128 error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched hardware-efficient-ansatz workload")
+    ap.add_argument("--batch-circuits", type=int, default=512)
     ap.add_argument("--calibration-passes", type=int, default=0,
                     help="extra in-place scale-by-1 passes (k_scale: one read and one write of the state), "
                          "the known-size kernel that hardware byte counters are scaled by")
@@ -134,6 +178,9 @@ def main() -> int:
     result_extra["qft_passes"] = st_q["passes"]
     log(f"QFT {n}: {st_q['gate_seconds']:.4f} s, {st_q['passes']} passes")
     sv.close()
+
+    if not args.no_batch:
+        result_extra["batch"] = run_batch(args.batch_circuits)
 
     cpu = None
     if not args.no_cpu:

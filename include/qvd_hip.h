@@ -164,6 +164,40 @@ int qvd_gate_matrix(const char *name, const double *params, uint32_t num_params,
 int qvd_gate_lower(const char *name, const double *params, uint32_t num_params, uint32_t *num_controls,
                    uint32_t *num_targets, double *matrix, uint32_t capacity);
 
+/* ---- Batches of circuits ------------------------------------------------
+ * Many circuits, each on its own state, run together: the engine plans every
+ * circuit into register-tile stages, then runs stage round r of all circuits
+ * in one kernel launch (one descriptor per circuit). Reset, the rounds and
+ * the expectation values are captured once in a HIP graph at creation and
+ * replayed by every qvd_batch_run. States smaller than one register tile
+ * (14 qubits in complex64, 13 in complex128) are widened with ancilla qubits
+ * in |0>, which no op touches. New in this engine: the reference runs such
+ * circuits one at a time (run_on, src/simulator.rs:355-426); the estimator
+ * path it serves is python/qevedo/qiskit.py:584-589. */
+typedef struct qvd_batch qvd_batch;
+
+typedef struct qvd_circuit {
+    uint32_t num_qubits;
+    const qvd_op *ops;
+    uint64_t num_ops;
+} qvd_circuit;
+
+/* Plans, uploads and captures `count` circuits. Every op must plan into
+ * register-tile stages (ops of at most 4 targets, after fusion); a circuit
+ * that needs a full-state pass is rejected with QVD_EUNSUPPORTED, and runs
+ * through qvd_sv_apply_ops instead. */
+int qvd_batch_create(const qvd_circuit *circuits, uint32_t count, int precision, const qvd_options *options,
+                     qvd_batch **out);
+int qvd_batch_destroy(qvd_batch *batch);
+/* Runs every circuit from |0...0> and writes <Z_q> (num_qubits values per
+ * circuit) to z and <Z_q Z_{q+1}> (num_qubits - 1 per circuit) to zz, both
+ * concatenated in circuit order, FP64. stats: gates, kernels, passes (stage
+ * rounds), stages (register-tile stage descriptors), gate_seconds (the graph
+ * replay, host clock). */
+int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats);
+/* Sizes of the z and zz arrays qvd_batch_run fills. */
+int qvd_batch_outputs(const qvd_batch *batch, uint64_t *num_z, uint64_t *num_zz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,29 @@ struct qvd_sv {
     }
 };
 
+// A batch: every circuit's state back to back in one allocation, the
+// register-tile programs of all stages, one launch table per stage round and
+// the HIP graph that replays reset, rounds and expectation values.
+struct qvd_batch {
+    int precision = QVD_C64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t count = 0;
+    uint64_t len = 0;            // amplitudes of all states
+    uint64_t num_z = 0, num_zz = 0;
+    uint64_t gates = 0, kernels = 0, stages = 0;
+    DBuf amps, ins, coefs, ldsops, ints, descs, block_start, basis;
+    DBuf edescs, seg_start, out_start, seg_sums, z, zz;
+    uint32_t rounds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~qvd_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 namespace qvd {
 namespace {
 
@@ -341,16 +364,21 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
         QVD_HIP(hipMemcpyAsync(dvlds.p, vlds.data(), vlds.size() * sizeof(StageOp), hipMemcpyHostToDevice, sv->stream));
     if (!vints.empty())
         QVD_HIP(hipMemcpyAsync(dvints.p, vints.data(), vints.size() * sizeof(uint32_t), hipMemcpyHostToDevice, sv->stream));
+    DBuf dvdescs(std::max<size_t>(vdescs.size(), 1) * sizeof(VStageDesc));
+    if (!vdescs.empty())
+        QVD_HIP(hipMemcpyAsync(dvdescs.p, vdescs.data(), vdescs.size() * sizeof(VStageDesc), hipMemcpyHostToDevice,
+                               sv->stream));
     const int cus = num_cus(sv->device);
     size_t si = 0, vi = 0;
     for (const Step &step : steps) {
         if (step.is_stage && step.stage.reg_tile) {
-            const VStageDesc &d = vdescs[vi++];
+            const VStageDesc &d = vdescs[vi];
             st.kernels += step.stage.ops.size();
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, (uint64_t)cus);
             timer.begin(timer.stage);
-            QVD_HIP(launch_vstage<T2>(s, d, dvins.as<VInstr>(), dvco.as<T2>(), dvlds.as<StageOp>(), dvints.as<uint32_t>(),
-                                      grid, sv->stream));
+            QVD_HIP(launch_vstage<T2>(s, dvdescs.as<VStageDesc>() + vi, 1, nullptr, dvins.as<VInstr>(), dvco.as<T2>(),
+                                      dvlds.as<StageOp>(), dvints.as<uint32_t>(), grid, sv->stream));
+            vi++;
             timer.end(timer.stage);
             st.stages++;
             st.passes++;
@@ -466,6 +494,152 @@ void count_plan(const std::vector<Step> &steps, qvd_stats &st) {
             st.kernels++;
         }
     }
+}
+
+// Builds a batch: plans each circuit on max(n, L) qubits (L = the register
+// tile), lowers its stages, lays out the round launch tables and the
+// expectation descriptors, uploads them and captures the graph.
+template <typename T2>
+void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, const qvd_options &opt_in) {
+    const VGeom vg = vstage_geom<T2>();
+    const uint32_t L = vg.L();
+    qvd_options opt = opt_in;
+    opt.tile_bits = 0;  // register tiles
+    std::vector<uint64_t> state_off(count);
+    std::vector<std::vector<VStageDesc>> stage_descs(count);
+    std::vector<VInstr> vins;
+    std::vector<C64> vco;
+    std::vector<StageOp> vlds;
+    std::vector<uint32_t> vints;
+    std::vector<ExpDesc> ed(count);
+    std::vector<uint64_t> seg_start(count + 1, 0);
+    std::vector<uint32_t> out_start(count + 1, 0);
+    uint64_t off = 0, seg_off = 0;
+    for (uint32_t c = 0; c < count; c++) {
+        const qvd_circuit &ci = circuits[c];
+        require(ci.num_qubits >= 1 && ci.num_qubits <= 30, "batched circuits have 1 to 30 qubits");
+        require(ci.num_ops == 0 || ci.ops != nullptr, "null ops");
+        // Ops must stay on the circuit's own qubits, not the ancillas.
+        for (uint64_t i = 0; i < ci.num_ops; i++) {
+            const qvd_op &o = ci.ops[i];
+            for (uint32_t j = 0; j < o.num_targets; j++)
+                require(o.targets && o.targets[j] < ci.num_qubits, "circuit " + std::to_string(c) + ": qubit out of range");
+            for (uint32_t j = 0; j < o.num_controls; j++)
+                require(o.controls && o.controls[j] < ci.num_qubits, "circuit " + std::to_string(c) + ": qubit out of range");
+        }
+        const uint32_t npad = std::max(ci.num_qubits, L);
+        std::vector<Step> steps = make_plan(npad, b->precision, ci.ops, ci.num_ops, opt);
+        PendingDiag pend(npad);
+        state_off[c] = off;
+        for (const Step &st : steps) {
+            if (!st.is_stage || !st.stage.reg_tile)
+                fail(QVD_EUNSUPPORTED, "circuit " + std::to_string(c) +
+                                           " needs a full-state pass; run it with qvd_sv_apply_ops");
+            VProgram p = build_vprogram(st.stage, npad, vg, pend, vco, vlds, vints, true);
+            require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
+            p.desc.instr_off = (uint32_t)vins.size();
+            p.desc.state_off = off;
+            vins.insert(vins.end(), p.instr.begin(), p.instr.end());
+            stage_descs[c].push_back(p.desc);
+            b->kernels += st.stage.ops.size();
+        }
+        b->gates += ci.num_ops;
+        ExpDesc &e = ed[c];
+        e = expect_desc(npad, ci.num_qubits);
+        e.state_off = off;
+        e.seg_off = seg_off;
+        e.z_off = (uint32_t)b->num_z;
+        e.zz_off = (uint32_t)b->num_zz;
+        seg_off += e.nsegs * 2 * e.B;
+        seg_start[c + 1] = seg_start[c] + e.nsegs;
+        out_start[c + 1] = out_start[c] + 2 * ci.num_qubits - 1;
+        b->num_z += ci.num_qubits;
+        b->num_zz += ci.num_qubits - 1;
+        off += uint64_t(1) << npad;
+    }
+    b->len = off;
+    // Round r: stage r of every circuit that has one; one workgroup per tile.
+    std::vector<VStageDesc> all_descs;
+    std::vector<uint32_t> all_bs;
+    struct Round {
+        uint32_t d0, nd, bs0;
+        unsigned grid;
+    };
+    std::vector<Round> rounds;
+    for (uint32_t r = 0;; r++) {
+        Round rd{(uint32_t)all_descs.size(), 0, (uint32_t)all_bs.size(), 0};
+        uint64_t blocks = 0;
+        all_bs.push_back(0);
+        for (uint32_t c = 0; c < count; c++) {
+            if (r >= stage_descs[c].size()) continue;
+            const VStageDesc &d = stage_descs[c][r];
+            all_descs.push_back(d);
+            blocks += d.num_tiles;
+            require(blocks < (uint64_t(1) << 31), "batch too large for one launch");
+            all_bs.push_back((uint32_t)blocks);
+            rd.nd++;
+        }
+        if (rd.nd == 0) {
+            all_bs.pop_back();
+            break;
+        }
+        rd.grid = (unsigned)blocks;
+        rounds.push_back(rd);
+        b->stages += rd.nd;
+    }
+    b->rounds = (uint32_t)rounds.size();
+    // Upload.
+    std::vector<T2> vco_t(vco.size());
+    for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
+    auto upload = [&](DBuf &buf, const void *src, size_t bytes) {
+        buf.alloc(std::max<size_t>(bytes, 16));
+        if (bytes) QVD_HIP(hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    upload(b->ins, vins.data(), vins.size() * sizeof(VInstr));
+    upload(b->coefs, vco_t.data(), vco_t.size() * sizeof(T2));
+    upload(b->ldsops, vlds.data(), vlds.size() * sizeof(StageOp));
+    upload(b->ints, vints.data(), vints.size() * sizeof(uint32_t));
+    upload(b->descs, all_descs.data(), all_descs.size() * sizeof(VStageDesc));
+    upload(b->block_start, all_bs.data(), all_bs.size() * sizeof(uint32_t));
+    upload(b->basis, state_off.data(), state_off.size() * sizeof(uint64_t));
+    upload(b->edescs, ed.data(), ed.size() * sizeof(ExpDesc));
+    upload(b->seg_start, seg_start.data(), seg_start.size() * sizeof(uint64_t));
+    upload(b->out_start, out_start.data(), out_start.size() * sizeof(uint32_t));
+    b->seg_sums.alloc(std::max<uint64_t>(seg_off, 1) * sizeof(double));
+    b->z.alloc(std::max<uint64_t>(b->num_z, 1) * sizeof(double));
+    b->zz.alloc(std::max<uint64_t>(b->num_zz, 1) * sizeof(double));
+    const uint64_t bytes = b->len * sizeof(T2);
+    const uint64_t cap = state_cap_bytes();
+    if (cap && bytes > cap) fail(QVD_ENOMEM, "batch states of " + std::to_string(bytes) + " bytes exceed QVD_MAX_STATE_BYTES");
+    b->amps.alloc(bytes);
+    // One eager run first (also sets the kernels' LDS attributes outside the
+    // capture), then the capture of the same sequence.
+    T2 *s = b->amps.as<T2>();
+    auto enqueue = [&]() {
+        launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
+        QVD_HIP(hipGetLastError());
+        for (const Round &rd : rounds)
+            QVD_HIP(launch_vstage<T2>(s, b->descs.as<VStageDesc>() + rd.d0, rd.nd, b->block_start.as<uint32_t>() + rd.bs0,
+                                      b->ins.as<VInstr>(), b->coefs.as<T2>(), b->ldsops.as<StageOp>(),
+                                      b->ints.as<uint32_t>(), rd.grid, b->stream));
+        launch_expect_batch<T2>(s, b->edescs.as<ExpDesc>(), count, b->seg_start.as<uint64_t>(), seg_start[count],
+                                b->out_start.as<uint32_t>(), out_start[count], b->seg_sums.as<double>(),
+                                b->z.as<double>(), b->zz.as<double>(), b->stream);
+        QVD_HIP(hipGetLastError());
+    };
+    enqueue();
+    QVD_HIP(hipStreamSynchronize(b->stream));
+    QVD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue();
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    QVD_HIP(hipStreamEndCapture(b->stream, &b->graph));
+    QVD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 }
 
 }  // namespace
@@ -873,4 +1047,67 @@ int qvd_sv_last_sample_seconds(const qvd_sv *sv, double *out) {
     });
 }
 
+
+int qvd_batch_create(const qvd_circuit *circuits, uint32_t count, int precision, const qvd_options *options,
+                     qvd_batch **out) {
+    return guard([&] {
+        require(out != nullptr, "null output handle");
+        *out = nullptr;
+        require(precision == QVD_C64 || precision == QVD_C128, "precision must be QVD_C64 or QVD_C128");
+        require(count >= 1 && circuits != nullptr, "a batch needs at least one circuit");
+        qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
+        if (options) opt = *options;
+        int devices = 0;
+        if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) fail(QVD_EHIP, "no HIP device is visible");
+        std::unique_ptr<qvd_batch> b(new qvd_batch());
+        b->precision = precision;
+        b->count = count;
+        QVD_HIP(hipGetDevice(&b->device));
+        QVD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        if (precision == QVD_C64) build_batch<float2>(b.get(), circuits, count, opt);
+        else build_batch<double2>(b.get(), circuits, count, opt);
+        *out = b.release();
+    });
+}
+
+int qvd_batch_destroy(qvd_batch *batch) {
+    return guard([&] {
+        if (!batch) return;
+        (void)hipSetDevice(batch->device);
+        (void)hipStreamSynchronize(batch->stream);
+        delete batch;
+    });
+}
+
+int qvd_batch_outputs(const qvd_batch *batch, uint64_t *num_z, uint64_t *num_zz) {
+    return guard([&] {
+        require(batch != nullptr, "null batch handle");
+        if (num_z) *num_z = batch->num_z;
+        if (num_zz) *num_zz = batch->num_zz;
+    });
+}
+
+int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats) {
+    return guard([&] {
+        require(batch != nullptr, "null batch handle");
+        QVD_HIP(hipSetDevice(batch->device));
+        const double t0 = now();
+        QVD_HIP(hipGraphLaunch(batch->exec, batch->stream));
+        QVD_HIP(hipStreamSynchronize(batch->stream));
+        const double t1 = now();
+        if (z && batch->num_z)
+            QVD_HIP(hipMemcpy(z, batch->z.p, batch->num_z * sizeof(double), hipMemcpyDeviceToHost));
+        if (zz && batch->num_zz)
+            QVD_HIP(hipMemcpy(zz, batch->zz.p, batch->num_zz * sizeof(double), hipMemcpyDeviceToHost));
+        if (stats) {
+            qvd_stats st{};
+            st.gates = batch->gates;
+            st.kernels = batch->kernels;
+            st.passes = batch->rounds;
+            st.stages = batch->stages;
+            st.gate_seconds = t1 - t0;
+            *stats = st;
+        }
+    });
+}
 }  // extern "C"

@@ -42,8 +42,26 @@ void launch_masked_norm(const T2 *s, uint64_t len, uint64_t mask, uint64_t val, 
 template <typename TA, typename TB>
 void launch_compare(const TA *a, const TB *b, uint64_t len, double *partial, unsigned blocks, hipStream_t st);
 uint32_t expect_segment_bits(uint32_t n);
+// <Z_q> and <Z_q Z_{q+1}> of one state among several: segment sums over
+// 2^B-amplitude segments, then one combine thread per output value.
+struct ExpDesc {
+    uint64_t state_off;  // first amplitude of the state
+    uint64_t seg_off;    // first double of its segment sums (2 B per segment)
+    uint64_t nsegs;      // segments
+    uint32_t B;          // segment bits
+    uint32_t n;          // qubits reported: n values of z, n - 1 of zz
+    uint32_t z_off, zz_off;
+};
+ExpDesc expect_desc(uint32_t state_qubits, uint32_t n);
 template <typename T2>
 void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, double *zz, hipStream_t st);
+template <typename T2>
+void launch_expect_batch(const T2 *s, const ExpDesc *descs, uint32_t ndesc, const uint64_t *seg_start,
+                         uint64_t total_segs, const uint32_t *out_start, uint32_t total_outs, double *seg_sums,
+                         double *z, double *zz, hipStream_t st);
+// Zero-fills `len` amplitudes and sets amplitude offs[i] to 1 for each state i.
+template <typename T2>
+void launch_batch_init(T2 *s, uint64_t len, const uint64_t *offs, uint32_t count, hipStream_t st);
 template <typename T2>
 void launch_collapse(T2 *s, uint64_t len, uint32_t qubit, uint32_t outcome, double keep, hipStream_t st);
 size_t sort_temp_bytes(uint64_t shots);

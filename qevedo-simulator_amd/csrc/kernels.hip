@@ -367,15 +367,39 @@ __global__ __launch_bounds__(RED_THREADS) void k_compare(const TA *__restrict__ 
 constexpr int EZ_TID_BITS = 8;   // bits from the thread index
 constexpr int EZ_MAX_M_BITS = 8; // bits from the per-thread loop
 
+// The descriptor of block b: `one` when there is a single state, else the
+// entry i of `descs` with start[i] <= b < start[i + 1] (binary search).
+template <typename S>
+__device__ __forceinline__ uint32_t find_range(const S *__restrict__ start, uint32_t count, uint64_t b) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)start[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <typename T2>
-__global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ s, uint32_t B,
-                                                         double *__restrict__ out) {
+__global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ s_all, ExpDesc one,
+                                                         const ExpDesc *__restrict__ descs, uint32_t ndesc,
+                                                         const uint64_t *__restrict__ seg_start,
+                                                         double *__restrict__ out_all) {
     __shared__ double scratch[4];
+    ExpDesc d = one;
+    uint64_t seg = blockIdx.x;
+    if (ndesc > 1) {
+        const uint32_t di = find_range(seg_start, ndesc, blockIdx.x);
+        d = descs[di];
+        seg = blockIdx.x - seg_start[di];
+    }
+    const T2 *s = s_all + d.state_off;
+    double *out = out_all + d.seg_off;
+    const uint32_t B = d.B;
     const uint32_t tb = B < EZ_TID_BITS ? B : EZ_TID_BITS;
     const uint32_t mb = B - tb;
     const uint32_t tid = threadIdx.x;
     const bool active = tid < (1u << tb);
-    const uint64_t seg = blockIdx.x;
     double T = 0, S[EZ_MAX_M_BITS], P[EZ_MAX_M_BITS];
 #pragma unroll
     for (int j = 0; j < EZ_MAX_M_BITS; j++) S[j] = P[j] = 0;
@@ -432,12 +456,23 @@ __global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ 
 
 // Combines the segment sums into z[q] = sum p (1 - 2 bit q) and
 // zz[q] = sum p (1 - 2 (bit q xor bit q+1)). One thread per output.
-__global__ void k_expect_combine(const double *__restrict__ seg_sums, uint64_t num_segs, uint32_t B,
-                                 uint32_t n, double *__restrict__ z, double *__restrict__ zz) {
-    const uint32_t w = threadIdx.x + blockIdx.x * blockDim.x;
+__global__ void k_expect_combine(const double *__restrict__ seg_all, ExpDesc one, const ExpDesc *__restrict__ descs,
+                                 uint32_t ndesc, const uint32_t *__restrict__ out_start, double *__restrict__ z,
+                                 double *__restrict__ zz) {
+    const uint32_t wg = threadIdx.x + blockIdx.x * blockDim.x;
+    ExpDesc d = one;
+    uint32_t w = wg;
+    if (ndesc > 1) {
+        if (wg >= out_start[ndesc]) return;
+        const uint32_t di = find_range(out_start, ndesc, wg);
+        d = descs[di];
+        w = wg - out_start[di];
+    }
+    const uint32_t n = d.n, B = d.B;
     if (w >= 2 * n - 1) return;
+    const double *seg_sums = seg_all + d.seg_off;
     double tot = 0, part = 0;
-    for (uint64_t g = 0; g < num_segs; g++) {
+    for (uint64_t g = 0; g < d.nsegs; g++) {
         const double *o = seg_sums + g * (2 * B);
         const double T = o[0];
         tot += T;
@@ -452,8 +487,19 @@ __global__ void k_expect_combine(const double *__restrict__ seg_sums, uint64_t n
             else if (((g >> (q - B)) ^ (g >> (q + 1 - B))) & 1) part += T;
         }
     }
-    if (w < n) z[w] = tot - 2 * part;
-    else zz[w - n] = tot - 2 * part;
+    if (w < n) z[d.z_off + w] = tot - 2 * part;
+    else zz[d.zz_off + w - n] = tot - 2 * part;
+}
+
+// Sets amplitude 0 of each state of a batch to 1 (after a zero fill).
+template <typename T2>
+__global__ void k_set_basis(T2 *__restrict__ s, const uint64_t *__restrict__ offs, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    T2 v;
+    v.x = 1;
+    v.y = 0;
+    s[offs[i]] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -634,13 +680,35 @@ void launch_masked_norm(const T2 *s, uint64_t len, uint64_t mask, uint64_t val, 
     k_sum_partials<<<1, 64, 0, st>>>(partial, blocks, out);
 }
 
+ExpDesc expect_desc(uint32_t state_qubits, uint32_t n) {
+    ExpDesc d{};
+    d.B = expect_segment_bits(state_qubits);
+    d.nsegs = uint64_t(1) << (state_qubits - d.B);
+    d.n = n;
+    return d;
+}
+
 template <typename T2>
 void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, double *zz, hipStream_t st) {
-    const uint32_t B = n < EZ_TID_BITS + EZ_MAX_M_BITS ? n : EZ_TID_BITS + EZ_MAX_M_BITS;
-    const uint64_t segs = uint64_t(1) << (n - B);
-    k_expect_segments<T2><<<(unsigned)segs, 256, 0, st>>>(s, B, seg_sums);
+    const ExpDesc d = expect_desc(n, n);
+    k_expect_segments<T2><<<(unsigned)d.nsegs, 256, 0, st>>>(s, d, nullptr, 1, nullptr, seg_sums);
     const uint32_t outs = 2 * n - 1;
-    k_expect_combine<<<(outs + 63) / 64, 64, 0, st>>>(seg_sums, segs, B, n, z, zz);
+    k_expect_combine<<<(outs + 63) / 64, 64, 0, st>>>(seg_sums, d, nullptr, 1, nullptr, z, zz);
+}
+
+template <typename T2>
+void launch_expect_batch(const T2 *s, const ExpDesc *descs, uint32_t ndesc, const uint64_t *seg_start,
+                         uint64_t total_segs, const uint32_t *out_start, uint32_t total_outs, double *seg_sums,
+                         double *z, double *zz, hipStream_t st) {
+    const ExpDesc none{};
+    k_expect_segments<T2><<<(unsigned)total_segs, 256, 0, st>>>(s, none, descs, ndesc, seg_start, seg_sums);
+    k_expect_combine<<<(total_outs + 63) / 64, 64, 0, st>>>(seg_sums, none, descs, ndesc, out_start, z, zz);
+}
+
+template <typename T2>
+void launch_batch_init(T2 *s, uint64_t len, const uint64_t *offs, uint32_t count, hipStream_t st) {
+    k_init_zero<T2><<<grid_for(len, 256), 256, 0, st>>>(s, len);  // amplitude 0 of the first state is 1
+    k_set_basis<T2><<<(count + 63) / 64, 64, 0, st>>>(s, offs, count);
 }
 
 template <typename TA, typename TB>
@@ -698,6 +766,9 @@ hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double
     template void launch_masked_norm<T2>(const T2 *, uint64_t, uint64_t, uint64_t, double *, unsigned, double *, \
                                          hipStream_t);                                                          \
     template void launch_expect_z<T2>(const T2 *, uint32_t, double *, double *, double *, hipStream_t);         \
+    template void launch_expect_batch<T2>(const T2 *, const ExpDesc *, uint32_t, const uint64_t *, uint64_t,    \
+                                          const uint32_t *, uint32_t, double *, double *, double *, hipStream_t); \
+    template void launch_batch_init<T2>(T2 *, uint64_t, const uint64_t *, uint32_t, hipStream_t);              \
     template void launch_collapse<T2>(T2 *, uint64_t, uint32_t, uint32_t, double, hipStream_t);                 \
     template hipError_t launch_sample<T2>(const T2 *, uint64_t, uint64_t, const double *, uint64_t,             \
                                           SampleScratch &, uint64_t *, hipStream_t);

@@ -78,13 +78,15 @@ struct VInstr {
     uint8_t pad[2];
 };
 
-// One register-tile stage as launched.
+// One register-tile stage as launched. A launch runs one or more of them
+// (distinct states of a batch), each on its own range of workgroups.
 struct VStageDesc {
     uint32_t L;
     uint32_t nglob;
     uint32_t num_instr;
     uint32_t instr_off;
     uint64_t num_tiles;
+    uint64_t state_off;              // first amplitude of the stage's state in the launch's buffer
     uint32_t local_pos[MAX_QUBITS];  // tile bit -> physical qubit (ascending)
     uint32_t glob_pos[MAX_QUBITS];   // tile-index bit -> physical qubit
     uint8_t lmap0[VMAX_L_FLEX];      // role map at load

@@ -19,6 +19,8 @@
 #include "vprog.h"
 #include "vstage.h"
 
+#include <atomic>
+
 // Constant address space: loads through these pointers are invariant, so
 // uniform ones go through the scalar cache (s_load) and never wait behind
 // the tile's vector loads.
@@ -499,7 +501,8 @@ __device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__res
 }  // namespace
 
 template <typename T2, int WB, int RB, int FB>
-__global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc d,
+__global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc *__restrict__ descs,
+                                                     uint32_t ndesc, const uint32_t *__restrict__ block_start,
                                                      const VInstr *__restrict__ prog, const T2 *__restrict__ coefs,
                                                      const StageOp *__restrict__ ldsops,
                                                      const uint32_t *__restrict__ ints) {
@@ -513,6 +516,24 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     const QVD_CONST StageOp *cops = (const QVD_CONST StageOp *)ldsops;
     const QVD_CONST uint32_t *cints = (const QVD_CONST uint32_t *)ints;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
+
+    // The workgroup's stage: descriptor di owns workgroups
+    // block_start[di] .. block_start[di + 1] - 1 (one descriptor: all).
+    uint32_t di = 0, bl = blockIdx.x, nb = gridDim.x;
+    if (ndesc > 1) {
+        const QVD_CONST uint32_t *cbs = (const QVD_CONST uint32_t *)block_start;
+        uint32_t lo = 0, hi = ndesc;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cbs[mid] <= blockIdx.x) lo = mid;
+            else hi = mid;
+        }
+        di = lo;
+        bl = blockIdx.x - cbs[di];
+        nb = cbs[di + 1] - cbs[di];
+    }
+    const QVD_CONST VStageDesc &d = ((const QVD_CONST VStageDesc *)descs)[di];
+    s += d.state_off;
 
     for (uint32_t i = tid; i < d.ccount; i += W) cst[i] = coefs[d.cbegin + i];
     const T2 *cbase = cst - d.cbegin;  // so cbase[v.coef + i] is the LDS copy
@@ -557,19 +578,19 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         }
     };
 
-    uint64_t t = blockIdx.x;
+    uint64_t t = bl;
     if (t < d.num_tiles) load(y, tile_base(t));
-    for (; t < d.num_tiles; t += gridDim.x) {
+    for (; t < d.num_tiles; t += nb) {
         const uint64_t base = tile_base(t);
 #pragma unroll
         for (int j = 0; j < R; j++) x[j] = y[j];
-        const uint64_t tn = t + gridDim.x;
+        const uint64_t tn = t + nb;
         if (tn < d.num_tiles) load(y, tile_base(tn));
 
         // Current role map: the thread's part of the tile index (fixed lane
         // bits, flexible lane bits, wave bits) and the register bits' tile
         // bits (uniform).
-        auto thread_index = [&](const uint8_t *lm, const uint8_t *wmp) {
+        auto thread_index = [&](const QVD_CONST uint8_t *lm, const QVD_CONST uint8_t *wmp) {
             uint32_t ti = lane & ((1u << FB) - 1);
 #pragma unroll
             for (int s_ = 0; s_ < (int)FL; s_++) ti |= ((lane >> (FB + s_)) & 1) << lm[s_];
@@ -716,20 +737,27 @@ size_t vstage_lds_bytes() {
 }
 
 template <typename T2>
-hipError_t launch_vstage(T2 *s, const VStageDesc &d, const VInstr *prog, const T2 *coefs, const StageOp *ldsops,
-                         const uint32_t *ints, unsigned grid, hipStream_t st) {
+hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const uint32_t *block_start,
+                         const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
+                         hipStream_t st) {
     const size_t lds = vstage_lds_bytes<T2>();
-    if constexpr (sizeof(T2) == 8) {
-        auto k = k_vstage<T2, 3, 5, 4>;
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    auto k = sizeof(T2) == 8 ? (void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
+                                         const T2 *, const StageOp *, const uint32_t *))k_vstage<T2, 3, 5, 4>
+                             : (void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
+                                         const T2 *, const StageOp *, const uint32_t *))k_vstage<T2, 3, 4, 3>;
+    // The LDS size attribute is set once per device, before any launch a
+    // caller may capture into a graph.
+    static std::atomic<uint64_t> attr_set{0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = uint64_t(1) << (dev & 63);
+    if (!(attr_set.load() & bit)) {
+        e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        k<<<grid, 64 << 3, lds, st>>>(s, d, prog, coefs, ldsops, ints);
-    } else {
-        auto k = k_vstage<T2, 3, 4, 3>;
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        k<<<grid, 64 << 3, lds, st>>>(s, d, prog, coefs, ldsops, ints);
+        attr_set.fetch_or(bit);
     }
+    k<<<grid, 64 << 3, lds, st>>>(s, descs, ndesc, block_start, prog, coefs, ldsops, ints);
     return hipGetLastError();
 }
 
@@ -737,9 +765,9 @@ template VGeom vstage_geom<float2>();
 template VGeom vstage_geom<double2>();
 template size_t vstage_lds_bytes<float2>();
 template size_t vstage_lds_bytes<double2>();
-template hipError_t launch_vstage<float2>(float2 *, const VStageDesc &, const VInstr *, const float2 *,
-                                          const StageOp *, const uint32_t *, unsigned, hipStream_t);
-template hipError_t launch_vstage<double2>(double2 *, const VStageDesc &, const VInstr *, const double2 *,
-                                           const StageOp *, const uint32_t *, unsigned, hipStream_t);
+template hipError_t launch_vstage<float2>(float2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
+                                          const float2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t);
+template hipError_t launch_vstage<double2>(double2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
+                                           const double2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t);
 
 }  // namespace qvd

@@ -22,12 +22,12 @@ import time
 import numpy as np
 
 from . import library
-from ._native import (NativeError, StateVector, device_info, gate_lower, gate_matrix, lower_gates, plan_ops)
+from ._native import (Batch, NativeError, StateVector, device_info, gate_lower, gate_matrix, lower_gates, plan_ops)
 from .circuit import Circuit
 
 __version__ = "0.1.18+mi355x"
 
-__all__ = ["Circuit", "NativeError", "RunResult", "StateVector", "choose_backend", "device_info", "gate_lower",
+__all__ = ["Batch", "Circuit", "NativeError", "RunResult", "StateVector", "choose_backend", "device_info", "gate_lower",
            "gate_matrix", "library", "lower_gates", "plan_ops", "run", "statevector"]
 
 BACKENDS = ("auto", "statevector")

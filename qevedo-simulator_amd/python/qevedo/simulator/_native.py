@@ -26,7 +26,8 @@ EXPORTS = (
     "qvd_sv_precision", "qvd_sv_set_zero", "qvd_sv_set_amplitudes", "qvd_sv_get_amplitudes", "qvd_sv_apply",
     "qvd_sv_apply_ops", "qvd_plan_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
     "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_compare", "qvd_sv_last_sample_seconds",
-    "qvd_gate_matrix", "qvd_gate_lower",
+    "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
+    "qvd_batch_outputs",
 )
 
 
@@ -53,6 +54,14 @@ class Options(C.Structure):
         ("tile_bits", C.c_int32),
         ("planner", C.c_uint32),
         ("flags", C.c_uint32),
+    ]
+
+
+class CircuitDesc(C.Structure):
+    _fields_ = [
+        ("num_qubits", C.c_uint32),
+        ("ops", C.POINTER(Op)),
+        ("num_ops", C.c_uint64),
     ]
 
 
@@ -109,6 +118,10 @@ def lib():
         L.qvd_sv_last_sample_seconds.argtypes = [vp, P(d)]
         L.qvd_gate_matrix.argtypes = [C.c_char_p, P(d), u32, P(d), u32, P(u32)]
         L.qvd_gate_lower.argtypes = [C.c_char_p, P(d), u32, P(u32), P(u32), P(d), u32]
+        L.qvd_batch_create.argtypes = [P(CircuitDesc), u32, C.c_int, P(Options), P(vp)]
+        L.qvd_batch_destroy.argtypes = [vp]
+        L.qvd_batch_run.argtypes = [vp, P(d), P(d), P(Stats)]
+        L.qvd_batch_outputs.argtypes = [vp, P(u64), P(u64)]
         for name in EXPORTS:
             getattr(L, name)  # fail early on a missing symbol
         _lib = L
@@ -306,3 +319,58 @@ class StateVector:
         v = C.c_double()
         _check(lib().qvd_sv_last_sample_seconds(self.h, C.byref(v)))
         return v.value
+
+
+class Batch:
+    """Many circuits, each on its own state, run together (qvd_batch_*).
+
+    circuits: a list of (num_qubits, lowered ops). run() returns, per circuit,
+    (<Z_q> for every qubit, <Z_q Z_{q+1}> for neighbouring qubits).
+    """
+
+    def __init__(self, circuits, precision: str = "single", max_fused_qubits: int = 4):
+        self.h = None
+        self.sizes = [int(n) for n, _ in circuits]
+        self.arrays = [ops if isinstance(ops, OpArray) else OpArray(ops) for _, ops in circuits]
+        cs = (CircuitDesc * len(self.sizes))()
+        for i, (n, arr) in enumerate(zip(self.sizes, self.arrays)):
+            cs[i] = CircuitDesc(n, arr.arr, arr.n)
+        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0)
+        h = C.c_void_p()
+        _check(lib().qvd_batch_create(cs, len(self.sizes), {"single": C64, "double": C128}[precision], C.byref(opt),
+                                      C.byref(h)))
+        self.h = h
+        nz, nzz = C.c_uint64(), C.c_uint64()
+        _check(lib().qvd_batch_outputs(self.h, C.byref(nz), C.byref(nzz)))
+        self.num_z, self.num_zz = nz.value, nzz.value
+        self.stats = {}
+
+    def run(self) -> list:
+        z = np.zeros(max(self.num_z, 1), dtype=np.float64)
+        zz = np.zeros(max(self.num_zz, 1), dtype=np.float64)
+        st = Stats()
+        _check(lib().qvd_batch_run(self.h, _p(z, C.c_double), _p(zz, C.c_double), C.byref(st)))
+        self.stats = st.as_dict()
+        out, iz, izz = [], 0, 0
+        for n in self.sizes:
+            out.append((z[iz:iz + n].copy(), zz[izz:izz + n - 1].copy()))
+            iz += n
+            izz += n - 1
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            lib().qvd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
