@@ -98,6 +98,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched hardware-efficient-ansatz workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned planner")
     ap.add_argument("--batch-circuits", type=int, default=512)
     ap.add_argument("--calibration-passes", type=int, default=0,
                     help="extra in-place scale-by-1 passes (k_scale: one read and one write of the state), "
@@ -150,6 +151,27 @@ def main() -> int:
     for _ in range(args.calibration_passes):
         sv.scale(1.0)
 
+    # The learned planner (a trained cost model in the tree): the same circuit
+    # with the plan variant the model predicts fastest.
+    learned_s = None
+    model_path = os.path.join(ROOT, "qevedo-simulator_amd", "share", "planner_model.json")
+    if os.path.exists(model_path) and not args.no_learned:
+        from qevedo.simulator import learned
+        net = learned.install(learned.load(model_path))
+        sv.set_zero()
+        st_l = sv.apply_ops(ops, args.fusion, args.tile_bits, planner=_native.PLANNER_LEARNED)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sv.set_zero()
+            st_l = sv.apply_ops(ops, args.fusion, args.tile_bits, planner=_native.PLANNER_LEARNED)
+        learned_s = (time.perf_counter() - t0) / args.steps
+        result_extra["learned_variant"] = st_l["plan_variant"]
+        result_extra["learned_plan_seconds"] = st_l["plan_seconds"]
+        log(f"learned planner: variant {st_l['plan_variant']}, {learned_s:.4f} s/circuit "
+            f"(heuristic {sec:.4f} s, plan {st_l['plan_seconds'] * 1e3:.1f} ms)")
+        _native.clear_planner_model()
+        net.close()
+
     # Sampling: 10^6 shots through the two-pass sampler.
     if args.shots:
         u = np.random.Generator(np.random.PCG64(7)).random(args.shots)
@@ -196,14 +218,17 @@ def main() -> int:
         cpu = {"value": cpu_s, "unit": "s", "cores": threads, "kind": "port",
                "sample": f"the same {n}-qubit depth-{depth} circuit once, complex64, fusion 4, regions of 2^14 blocks"}
 
+    # The headline is the learned planner's time when a trained model ships
+    # in the tree (the heuristic's time is in "plans"), else the heuristic's.
+    value = learned_s if learned_s is not None else sec
     out = {
         "metric": "30q random-circuit s (learned vs heuristic plan); HBM TB/s; err<=1e-5 vs FP64",
-        "value": sec,
+        "value": value,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
+        "ms_per_step": value * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -212,12 +237,12 @@ def main() -> int:
         "config": {"workload": f"random circuit, {n} qubits, depth {depth} (sqrt X/Y/W with no repeats + CZ brick), "
                                f"complex64 from |0...0>", "qubits": n, "depth": depth, "gates": len(gates),
                    "max_fused_qubits": args.fusion, "tile_bits": args.tile_bits or "register tile, 2^14 amplitudes",
-                   "planner": "heuristic"},
+                   "planner": "learned" if learned_s is not None else "heuristic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
         "error": error,
-        "plans": {"heuristic": sec, "learned": None},
+        "plans": {"heuristic": sec, "learned": learned_s},
     }
     out.update(result_extra)
     print(json.dumps(out), flush=True)

@@ -65,7 +65,15 @@ typedef struct qvd_options {
     int32_t tile_bits;         /* qubits per LDS-resident tile; <0 = no blocking, 0 = default */
     uint32_t planner;          /* QVD_PLANNER_* */
     uint32_t flags;            /* QVD_FLAG_* */
+    uint32_t variant;          /* heuristic plan variant, 0..QVD_PLAN_VARIANTS-1 (0 = the heuristic);
+                                  QVD_PLANNER_LEARNED chooses it itself */
 } qvd_options;
+
+/* Plan variants (blocking.rs's lookahead with a reduced capacity, a greedy
+ * choice of the next tile, register slots for the most-used qubits first)
+ * and the per-stage features the learned planner scores them by. */
+#define QVD_PLAN_VARIANTS 16
+#define QVD_PLAN_FEATURES 24
 
 /* qvd_options.flags: time every kernel launch with HIP events on the
  * handle's stream and report the sums in qvd_stats. */
@@ -82,6 +90,7 @@ typedef struct qvd_stats {
     double plan_seconds;          /* host planning inside gate_seconds */
     double stage_kernel_seconds;  /* sum of tile-stage kernel durations (QVD_FLAG_TIMING) */
     double other_kernel_seconds;  /* sum of full-state op kernel durations (QVD_FLAG_TIMING) */
+    uint64_t plan_variant;        /* the plan variant that ran */
 } qvd_stats;
 
 const char *qvd_last_error(void);
@@ -163,6 +172,42 @@ int qvd_gate_matrix(const char *name, const double *params, uint32_t num_params,
  * and the matrix of the gate applied to the remaining qubits. */
 int qvd_gate_lower(const char *name, const double *params, uint32_t num_params, uint32_t *num_controls,
                    uint32_t *num_targets, double *matrix, uint32_t capacity);
+
+/* Features of the register-tile stages of the plan qvd_sv_apply_ops would
+ * run with these options, QVD_PLAN_FEATURES floats per stage (row-major),
+ * in launch order; `capacity` counts rows. New in this engine: the inputs of
+ * the learned planner's cost model (the reference plans by a fixed rule,
+ * src/blocking.rs:150-302). */
+int qvd_plan_features(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops,
+                      const qvd_options *options, float *features, uint64_t capacity, uint64_t *rows);
+/* Kernel time of each register-tile stage of the last qvd_sv_apply_ops run
+ * with QVD_FLAG_TIMING, in launch order (seconds, HIP events). */
+int qvd_sv_stage_seconds(const qvd_sv *sv, double *seconds, uint64_t capacity, uint64_t *count);
+
+/* ---- The learned planner's cost model ------------------------------------
+ * A 3-layer perceptron (inputs -> hidden -> hidden -> 1, ReLU, FP32) whose
+ * forward, backward and Adam steps are HIP kernels of this library.
+ * Parameters: W1[hidden x inputs], b1[hidden], W2[hidden x hidden],
+ * b2[hidden], W3[hidden], b3 (row-major). New in this engine. */
+typedef struct qvd_mlp qvd_mlp;
+int qvd_mlp_create(uint32_t inputs, uint32_t hidden, qvd_mlp **out);
+int qvd_mlp_destroy(qvd_mlp *mlp);
+int qvd_mlp_num_params(const qvd_mlp *mlp, uint64_t *count);
+int qvd_mlp_set_params(qvd_mlp *mlp, const float *params, uint64_t count);
+int qvd_mlp_get_params(const qvd_mlp *mlp, float *params, uint64_t count);
+/* y[r] for `rows` rows of x (row-major, inputs floats each). */
+int qvd_mlp_forward(qvd_mlp *mlp, const float *x, uint64_t rows, float *y);
+/* Gradient of the mean squared error (y - t)^2 over the rows with respect to
+ * every parameter, and the loss. */
+int qvd_mlp_gradient(qvd_mlp *mlp, const float *x, const float *t, uint64_t rows, float *grad, double *loss);
+/* Adam (lr, betas 0.9 / 0.999) on shuffled minibatches; loss = final MSE. */
+int qvd_mlp_train(qvd_mlp *mlp, const float *x, const float *t, uint64_t rows, uint32_t epochs, uint32_t batch,
+                  float lr, uint64_t seed, double *loss);
+/* Installs a copy of `mlp` (QVD_PLAN_FEATURES inputs) as the cost model of
+ * QVD_PLANNER_LEARNED: the model sees (feature - mean) / scale and predicts
+ * (log(seconds per amplitude) - y_mean) / y_scale. NULL removes the model,
+ * and the learned planner then falls back to the heuristic. */
+int qvd_planner_set_model(const qvd_mlp *mlp, const float *mean, const float *scale, float y_mean, float y_scale);
 
 /* ---- Batches of circuits ------------------------------------------------
  * Many circuits, each on its own state, run together: the engine plans every

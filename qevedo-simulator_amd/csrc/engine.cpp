@@ -11,6 +11,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "mlp.h"
 #include "plan.h"
 #include "vprog.h"
 #include "vstage.h"
@@ -81,6 +82,7 @@ struct qvd_sv {
     hipStream_t stream = nullptr;
     DBuf scratch;  // small reduction buffers
     double last_sample_seconds = 0;
+    std::vector<double> stage_seconds;  // register-tile stage kernels of the last timed apply_ops
 
     size_t elem() const { return precision == QVD_C64 ? sizeof(float2) : sizeof(double2); }
     void bind() const { QVD_HIP(hipSetDevice(device)); }
@@ -112,6 +114,11 @@ struct qvd_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
     }
+};
+
+struct qvd_mlp {
+    qvd::Mlp net;
+    qvd_mlp(uint32_t in, uint32_t hidden) : net(in, hidden) {}
 };
 
 namespace qvd {
@@ -370,12 +377,14 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
                                sv->stream));
     const int cus = num_cus(sv->device);
     size_t si = 0, vi = 0;
+    std::vector<char> reg_launch;  // per timed stage launch: register-tile or not
     for (const Step &step : steps) {
         if (step.is_stage && step.stage.reg_tile) {
             const VStageDesc &d = vdescs[vi];
             st.kernels += step.stage.ops.size();
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, (uint64_t)cus);
             timer.begin(timer.stage);
+            reg_launch.push_back(1);
             QVD_HIP(launch_vstage<T2>(s, dvdescs.as<VStageDesc>() + vi, 1, nullptr, dvins.as<VInstr>(), dvco.as<T2>(),
                                       dvlds.as<StageOp>(), dvints.as<uint32_t>(), grid, sv->stream));
             vi++;
@@ -391,6 +400,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             const uint64_t want = (uint64_t)cus * per_cu * 2;
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, want);
             timer.begin(timer.stage);
+            reg_launch.push_back(0);
             launch_stage<T2>(s, d, dops.as<StageOp>(), dcoef.as<T2>(), grid, sv->stream);
             QVD_HIP(hipGetLastError());
             timer.end(timer.stage);
@@ -408,6 +418,13 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
     if (timing) {
         st.stage_kernel_seconds = LaunchTimer::total(timer.stage);
         st.other_kernel_seconds = LaunchTimer::total(timer.other);
+        sv->stage_seconds.clear();
+        for (size_t i = 0; i < timer.stage.size(); i++) {
+            if (!reg_launch[i]) continue;
+            float ms = 0;
+            QVD_HIP(hipEventElapsedTime(&ms, timer.stage[i].first, timer.stage[i].second));
+            sv->stage_seconds.push_back(ms * 1e-3);
+        }
     }
 }
 
@@ -417,9 +434,126 @@ uint64_t state_cap_bytes() {
     return std::strtoull(e, nullptr, 10);
 }
 
-// Validates the ops and plans them (heuristic planner) for a state of n
-// qubits in `precision`.
-std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64_t num_ops, const qvd_options &opt) {
+// Splits register-tile stages whose coefficients would exceed the LDS
+// budget of one stage (each part is a pass).
+std::vector<Step> split_for_coefficients(std::vector<Step> steps, uint32_t n, int precision, const VGeom &vg) {
+    const size_t budget = VCONST_BYTES / (precision == QVD_C64 ? sizeof(float2) : sizeof(double2));
+    const size_t scale = (64u << vg.w_bits) + (1u << vg.r_bits) + 2 * n;
+    auto entries = [](const Op &op) -> size_t {
+        const size_t k = op.targets.size();
+        if (op.diagonal) return size_t(1) << k;
+        return size_t(1) << (2 * k);
+    };
+    std::vector<Step> out;
+    for (Step &s : steps) {
+        if (!s.is_stage) {
+            out.push_back(std::move(s));
+            continue;
+        }
+        Step cur;
+        cur.stage.local = s.stage.local;
+        cur.stage.reg_tile = true;
+        cur.stage.role_policy = s.stage.role_policy;
+        size_t used = scale;
+        for (Op &op : s.stage.ops) {
+            const size_t e = entries(op);
+            if (used + e > budget && !cur.stage.ops.empty()) {
+                out.push_back(cur);
+                cur.stage.ops.clear();
+                used = scale;
+            }
+            used += e;
+            cur.stage.ops.push_back(std::move(op));
+        }
+        if (!cur.stage.ops.empty()) out.push_back(std::move(cur));
+    }
+    return out;
+}
+
+// Features of each register-tile stage of a plan, in launch order: what the
+// stage's program does per tile (counts of each instruction kind, in
+// log(1 + count)), the state size and the precision.
+std::vector<float> plan_features(const std::vector<Step> &steps, uint32_t n, int precision) {
+    const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    std::vector<float> rows;
+    PendingDiag pend(n);
+    std::vector<C64> co;
+    std::vector<StageOp> lds;
+    std::vector<uint32_t> ints;
+    for (const Step &step : steps) {
+        if (!step.is_stage || !step.stage.reg_tile) continue;
+        const size_t lds0 = lds.size();
+        VProgram p = build_vprogram(step.stage, n, vg, pend, co, lds, ints, true);
+        double f[QVD_PLAN_FEATURES] = {};
+        f[0] = n / 32.0;
+        f[1] = precision == QVD_C128 ? 1.0 : 0.0;
+        for (const VInstr &v : p.instr) {
+            const uint32_t op = v.op & VOP_MASK;
+            const bool ctrl = (v.op & VF_CTRL) != 0;
+            if (v.op & VF_GCTRL) f[22]++;
+            f[23]++;
+            switch (op) {
+                case V_U1R:
+                    if (ctrl || v.form == FORM_FULL) f[5]++;
+                    else if (v.form == FORM_UNIT_ROT) f[2]++;
+                    else if (v.form == FORM_UNIT_DIAG) f[3]++;
+                    else f[4]++;
+                    break;
+                case V_U1L:
+                    if (ctrl) f[9]++;
+                    else if (v.slot < 4) f[6]++;
+                    else if (v.form == FORM_UNIT_ROT) f[7]++;
+                    else f[8]++;
+                    break;
+                case V_DIAG:
+                    f[10]++;
+                    if (v.threaded) f[11]++;
+                    break;
+                case V_SIGNS:
+                    f[12]++;
+                    f[13] += v.k;
+                    if (v.neg) f[14]++;
+                    break;
+                case V_PHASES:
+                    f[15]++;
+                    f[16] += v.k;
+                    if (v.form == 1) f[17]++;
+                    break;
+                case V_XCHG: f[18]++; break;
+                case V_LDSOP:
+                    f[19]++;
+                    f[20] += double(1u << (2 * lds[v.aux].k));
+                    break;
+                case V_SCALE:
+                    if (v.form != 1) f[21]++;
+                    break;
+                default: break;
+            }
+        }
+        (void)lds0;
+        for (int k = 2; k < QVD_PLAN_FEATURES; k++) f[k] = std::log1p(f[k]);
+        for (int k = 0; k < QVD_PLAN_FEATURES; k++) rows.push_back((float)f[k]);
+    }
+    return rows;
+}
+
+// The planner's cost model, when one is installed (qvd_planner_set_model):
+// predicted log(seconds per amplitude) of a register-tile stage from its
+// features, with the feature and target normalisation of its training set.
+struct PlannerModel {
+    std::mutex mu;
+    std::unique_ptr<Mlp> net;
+    std::vector<float> mean, scale;
+    float ymean = 0, yscale = 1;
+};
+PlannerModel g_model;
+
+// Validates the ops and plans them for a state of n qubits in `precision`:
+// the heuristic (options.variant picks a variant of it), or, with
+// QVD_PLANNER_LEARNED and a cost model installed, the variant whose stages
+// the model predicts to run fastest (the heuristic when no model is set).
+std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64_t num_ops, const qvd_options &opt,
+                            uint32_t *chosen = nullptr) {
     require(num_ops == 0 || ops != nullptr, "null ops");
     std::vector<Op> v;
     v.reserve(num_ops);
@@ -430,6 +564,7 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
             fail(QVD_EUNSUPPORTED, "ops on more than " + std::to_string(MAX_GATE_QUBITS) + " target qubits are not supported");
         v.push_back(std::move(op));
     }
+    if (chosen) *chosen = 0;
     if (opt.tile_bits < 0) return plan_unblocked(v, opt.max_fused_qubits);
     TileConfig tc = default_tiles(precision);
     const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
@@ -448,40 +583,54 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
                 "tile_bits out of range (1..14 for complex64, 1..13 for complex128)");
         p.contig = std::min(tc.contig, p.tile_bits);
     }
-    std::vector<Step> steps = plan_heuristic(v, p);
-    if (!p.reg_tile) return steps;
-    // Each register-tile stage copies its coefficients to LDS: split stages
-    // whose coefficients would exceed the budget (each part is a pass).
-    const size_t budget = VCONST_BYTES / (precision == QVD_C64 ? sizeof(float2) : sizeof(double2));
-    const size_t scale = (64u << vg.w_bits) + (1u << vg.r_bits) + 2 * n;
-    auto entries = [](const Op &op) -> size_t {
-        const size_t k = op.targets.size();
-        if (op.diagonal) return size_t(1) << k;
-        return size_t(1) << (2 * k);
-    };
-    std::vector<Step> out;
-    for (Step &s : steps) {
-        if (!s.is_stage) {
-            out.push_back(std::move(s));
-            continue;
-        }
-        Step cur;
-        cur.stage.local = s.stage.local;
-        cur.stage.reg_tile = true;
-        size_t used = scale;
-        for (Op &op : s.stage.ops) {
-            const size_t e = entries(op);
-            if (used + e > budget && !cur.stage.ops.empty()) {
-                out.push_back(cur);
-                cur.stage.ops.clear();
-                used = scale;
-            }
-            used += e;
-            cur.stage.ops.push_back(std::move(op));
-        }
-        if (!cur.stage.ops.empty()) out.push_back(std::move(cur));
+    require(opt.variant < PLAN_VARIANTS, "plan variant out of range");
+    p.variant = opt.variant;
+    if (!p.reg_tile) return plan_heuristic(v, p);
+    std::unique_lock<std::mutex> lock(g_model.mu);
+    if (opt.planner != QVD_PLANNER_LEARNED || !g_model.net) {
+        lock.unlock();
+        if (chosen) *chosen = p.variant;
+        return split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
     }
-    return out;
+    // Learned: every variant, scored by the cost model in one batched
+    // forward pass on the device.
+    const uint32_t F = QVD_PLAN_FEATURES;
+    std::vector<std::vector<Step>> plans(PLAN_VARIANTS);
+    std::vector<float> rows;
+    std::vector<size_t> first(PLAN_VARIANTS + 1, 0);
+    std::vector<double> fixed(PLAN_VARIANTS, 0.0);
+    const double amps = std::ldexp(1.0, (int)n);
+    const double elem = precision == QVD_C64 ? 8.0 : 16.0;
+    for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
+        p.variant = var;
+        plans[var] = split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
+        std::vector<float> r = plan_features(plans[var], n, precision);
+        for (size_t k = 0; k < r.size(); k++) {
+            const size_t j = k % F;
+            r[k] = (r[k] - g_model.mean[j]) / g_model.scale[j];
+        }
+        rows.insert(rows.end(), r.begin(), r.end());
+        first[var + 1] = rows.size() / F;
+        // A full-state pass outside a stage: one read and one write at the
+        // HBM copy rate.
+        for (const Step &s : plans[var])
+            if (!s.is_stage) fixed[var] += 2 * amps * elem / 4.5e12;
+    }
+    std::vector<float> pred(rows.size() / F);
+    g_model.net->forward(rows.data(), pred.size(), pred.data());
+    uint32_t best = 0;
+    double best_t = 0;
+    for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
+        double t = fixed[var];
+        for (size_t k = first[var]; k < first[var + 1]; k++)
+            t += std::exp(g_model.ymean + g_model.yscale * pred[k]) * amps;
+        if (var == 0 || t < best_t) {
+            best = var;
+            best_t = t;
+        }
+    }
+    if (chosen) *chosen = best;
+    return std::move(plans[best]);
 }
 
 void count_plan(const std::vector<Step> &steps, qvd_stats &st) {
@@ -801,10 +950,12 @@ int qvd_plan_ops(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t
         qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
         if (options) opt = *options;
         const double t0 = now();
-        std::vector<Step> steps = make_plan(num_qubits, precision, ops, num_ops, opt);
+        uint32_t variant = 0;
+        std::vector<Step> steps = make_plan(num_qubits, precision, ops, num_ops, opt, &variant);
         qvd_stats st{};
         st.gates = num_ops;
         st.plan_seconds = now() - t0;
+        st.plan_variant = variant;
         count_plan(steps, st);
         if (stats) *stats = st;
     });
@@ -816,10 +967,12 @@ int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_
         const double t0 = now();
         qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
         if (options) opt = *options;
-        std::vector<Step> steps = make_plan(sv->n, sv->precision, ops, num_ops, opt);
+        uint32_t variant = 0;
+        std::vector<Step> steps = make_plan(sv->n, sv->precision, ops, num_ops, opt, &variant);
         const double t1 = now();
         qvd_stats st{};
         st.gates = num_ops;
+        st.plan_variant = variant;
         sv->bind();
         with_type(sv, [&](auto *s) {
             run_plan(sv, s, steps, st, (opt.flags & QVD_FLAG_TIMING) != 0);
@@ -1108,6 +1261,120 @@ int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats) {
             st.gate_seconds = t1 - t0;
             *stats = st;
         }
+    });
+}
+
+int qvd_plan_features(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops,
+                      const qvd_options *options, float *features, uint64_t capacity, uint64_t *rows) {
+    return guard([&] {
+        require(precision == QVD_C64 || precision == QVD_C128, "precision must be QVD_C64 or QVD_C128");
+        require(num_qubits <= 34, "a state vector of more than 34 qubits does not fit in memory");
+        require(rows != nullptr, "null output");
+        qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0, 0};
+        if (options) opt = *options;
+        std::vector<Step> steps = make_plan(num_qubits, precision, ops, num_ops, opt);
+        std::vector<float> f = plan_features(steps, num_qubits, precision);
+        const uint64_t r = f.size() / QVD_PLAN_FEATURES;
+        *rows = r;
+        require(r <= capacity || features == nullptr, "feature buffer too small");
+        if (features) std::copy(f.begin(), f.end(), features);
+    });
+}
+
+int qvd_sv_stage_seconds(const qvd_sv *sv, double *seconds, uint64_t capacity, uint64_t *count) {
+    return guard([&] {
+        check_handle(sv);
+        require(count != nullptr, "null output");
+        *count = sv->stage_seconds.size();
+        if (!seconds) return;
+        require(sv->stage_seconds.size() <= capacity, "buffer too small");
+        std::copy(sv->stage_seconds.begin(), sv->stage_seconds.end(), seconds);
+    });
+}
+
+int qvd_mlp_create(uint32_t inputs, uint32_t hidden, qvd_mlp **out) {
+    return guard([&] {
+        require(out != nullptr, "null output handle");
+        *out = nullptr;
+        *out = new qvd_mlp(inputs, hidden);
+    });
+}
+
+int qvd_mlp_destroy(qvd_mlp *mlp) {
+    return guard([&] { delete mlp; });
+}
+
+int qvd_mlp_num_params(const qvd_mlp *mlp, uint64_t *count) {
+    return guard([&] {
+        require(mlp != nullptr && count != nullptr, "null argument");
+        *count = mlp->net.shape().params();
+    });
+}
+
+int qvd_mlp_set_params(qvd_mlp *mlp, const float *params, uint64_t count) {
+    return guard([&] {
+        require(mlp != nullptr && params != nullptr, "null argument");
+        require(count == mlp->net.shape().params(), "parameter count does not match the network");
+        mlp->net.set_params(params);
+    });
+}
+
+int qvd_mlp_get_params(const qvd_mlp *mlp, float *params, uint64_t count) {
+    return guard([&] {
+        require(mlp != nullptr && params != nullptr, "null argument");
+        require(count == mlp->net.shape().params(), "parameter count does not match the network");
+        mlp->net.get_params(params);
+    });
+}
+
+int qvd_mlp_forward(qvd_mlp *mlp, const float *x, uint64_t rows, float *y) {
+    return guard([&] {
+        require(mlp != nullptr && (rows == 0 || (x != nullptr && y != nullptr)), "null argument");
+        require(rows < (uint64_t(1) << 31), "too many rows");
+        mlp->net.forward(x, rows, y);
+    });
+}
+
+int qvd_mlp_gradient(qvd_mlp *mlp, const float *x, const float *t, uint64_t rows, float *grad, double *loss) {
+    return guard([&] {
+        require(mlp != nullptr && x != nullptr && t != nullptr && grad != nullptr, "null argument");
+        require(rows >= 1 && rows < (uint64_t(1) << 31), "rows out of range");
+        const double l = mlp->net.gradient(x, t, rows, grad);
+        if (loss) *loss = l;
+    });
+}
+
+int qvd_mlp_train(qvd_mlp *mlp, const float *x, const float *t, uint64_t rows, uint32_t epochs, uint32_t batch,
+                  float lr, uint64_t seed, double *loss) {
+    return guard([&] {
+        require(mlp != nullptr && x != nullptr && t != nullptr, "null argument");
+        require(rows >= 1 && rows < (uint64_t(1) << 31) && batch >= 1, "rows or batch out of range");
+        require(lr > 0, "learning rate must be positive");
+        const double l = mlp->net.train(x, t, rows, epochs, batch, lr, seed);
+        if (loss) *loss = l;
+    });
+}
+
+int qvd_planner_set_model(const qvd_mlp *mlp, const float *mean, const float *scale, float y_mean, float y_scale) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lock(g_model.mu);
+        if (!mlp) {
+            g_model.net.reset();
+            return;
+        }
+        require(mean != nullptr && scale != nullptr, "null normalisation");
+        require(mlp->net.shape().inputs == QVD_PLAN_FEATURES, "the cost model must take QVD_PLAN_FEATURES inputs");
+        for (int k = 0; k < QVD_PLAN_FEATURES; k++) require(scale[k] > 0, "feature scales must be positive");
+        require(y_scale > 0, "target scale must be positive");
+        std::vector<float> p(mlp->net.shape().params());
+        mlp->net.get_params(p.data());
+        std::unique_ptr<Mlp> net(new Mlp(mlp->net.shape().inputs, mlp->net.shape().hidden));
+        net->set_params(p.data());
+        g_model.net = std::move(net);
+        g_model.mean.assign(mean, mean + QVD_PLAN_FEATURES);
+        g_model.scale.assign(scale, scale + QVD_PLAN_FEATURES);
+        g_model.ymean = y_mean;
+        g_model.yscale = y_scale;
     });
 }
 }  // extern "C"

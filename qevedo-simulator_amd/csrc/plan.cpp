@@ -162,6 +162,7 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
             Step s;
             s.stage.local = local;
             s.stage.reg_tile = p.reg_tile;
+            s.stage.role_policy = (p.variant >> 3) & 1;
             if (p.reg_tile) {
                 // The register-tile program merges 1-qubit runs itself and
                 // keeps gates in unit form; wider fusion would only add
@@ -181,24 +182,77 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
         if (remaining == 0) break;
         // Lookahead (src/blocking.rs:181-226): the qubits the coming ops
         // need, within the tile's free capacity.
-        const uint32_t capacity = L - contig;
-        std::vector<uint32_t> wanted;
-        extract(ops, done, [&](const Op &op) {
-            if (op.diagonal) return true;
-            if (op.targets.size() > p.max_stage_op) return false;
-            std::vector<uint32_t> fresh;
-            for (uint32_t q : op.targets)
-                if (q >= contig && !contains(wanted, q)) fresh.push_back(q);
-            if (wanted.size() + fresh.size() > capacity) return false;
-            wanted.insert(wanted.end(), fresh.begin(), fresh.end());
-            return true;
-        });
-        std::vector<uint32_t> next;
-        for (uint32_t q = 0; q < contig; q++) next.push_back(q);
-        for (uint32_t q : wanted) next.push_back(q);
-        for (uint32_t q = contig; q < n && next.size() < L; q++)
-            if (!contains(next, q)) next.push_back(q);
-        std::sort(next.begin(), next.end());
+        const uint32_t full_cap = L - contig;
+        auto wanted_set = [&](uint32_t capacity) {
+            std::vector<uint32_t> wanted;
+            extract(ops, done, [&](const Op &op) {
+                if (op.diagonal) return true;
+                if (op.targets.size() > p.max_stage_op) return false;
+                std::vector<uint32_t> fresh;
+                for (uint32_t q : op.targets)
+                    if (q >= contig && !contains(wanted, q)) fresh.push_back(q);
+                if (wanted.size() + fresh.size() > capacity) return false;
+                wanted.insert(wanted.end(), fresh.begin(), fresh.end());
+                return true;
+            });
+            return wanted;
+        };
+        auto tile_of = [&](const std::vector<uint32_t> &wanted) {
+            std::vector<uint32_t> next;
+            for (uint32_t q = 0; q < contig; q++) next.push_back(q);
+            for (uint32_t q : wanted) next.push_back(q);
+            for (uint32_t q = contig; q < n && next.size() < L; q++)
+                if (!contains(next, q)) next.push_back(q);
+            std::sort(next.begin(), next.end());
+            return next;
+        };
+        const uint32_t cut = std::min<uint32_t>(p.variant & 3, full_cap > 1 ? full_cap - 1 : 0);
+        std::vector<uint32_t> next = tile_of(wanted_set(full_cap - cut));
+        if (p.variant & 4) {
+            // The candidate tile that runs the most pending ops next.
+            auto runnable_count = [&](const std::vector<uint32_t> &tile) {
+                std::vector<char> loc(n, 0);
+                for (uint32_t q : tile) loc[q] = 1;
+                return extract(ops, done, [&](const Op &op) {
+                           if (op.diagonal) return true;
+                           if (op.targets.size() > p.max_stage_op) return false;
+                           for (uint32_t q : op.targets)
+                               if (!loc[q]) return false;
+                           return true;
+                       }).size();
+            };
+            std::vector<std::vector<uint32_t>> cands;
+            for (uint32_t c = full_cap; c >= 1 && c + 4 > full_cap; c--) cands.push_back(tile_of(wanted_set(c)));
+            // the most-targeted qubits among the next pending dense ops
+            std::vector<uint32_t> count(n, 0), first(n, UINT32_MAX);
+            uint32_t seen = 0;
+            for (size_t i = 0; i < ops.size() && seen < 8 * L; i++) {
+                if (done[i] || ops[i].diagonal) continue;
+                for (uint32_t q : ops[i].targets)
+                    if (q >= contig) {
+                        count[q]++;
+                        first[q] = std::min(first[q], seen);
+                    }
+                seen++;
+            }
+            std::vector<uint32_t> byuse;
+            for (uint32_t q = contig; q < n; q++)
+                if (count[q]) byuse.push_back(q);
+            std::stable_sort(byuse.begin(), byuse.end(), [&](uint32_t a, uint32_t b) {
+                return count[a] != count[b] ? count[a] > count[b] : first[a] < first[b];
+            });
+            if (byuse.size() > full_cap) byuse.resize(full_cap);
+            cands.push_back(tile_of(byuse));
+            size_t best = next == local ? 0 : runnable_count(next);
+            for (const auto &cnd : cands) {
+                if (cnd == local) continue;
+                const size_t k = runnable_count(cnd);
+                if (k > best) {
+                    best = k;
+                    next = cnd;
+                }
+            }
+        }
         bool same = next == local;
         if (same) {
             // Nothing new can be brought in (an op wider than the tile's

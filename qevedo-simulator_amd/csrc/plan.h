@@ -29,6 +29,7 @@ struct Stage {
     std::vector<uint32_t> local;  // tile bit j -> physical qubit local[j], ascending
     std::vector<Op> ops;          // program order (fused for LDS-tile stages)
     bool reg_tile = false;        // run by the register-tile kernel (ops unfused)
+    uint32_t role_policy = 0;     // register-tile programs: 0 = bits in order of need, 1 = most-used bits first
 };
 
 // One step of a plan: an LDS-tile stage, or a single op applied as its own
@@ -46,7 +47,16 @@ struct PlanParams {
     uint32_t max_fused = 4;
     uint32_t max_stage_op = 4;  // widest dense op applied inside a tile
     bool reg_tile = false;      // stages for the register-tile kernel: ops kept unfused
+    // Plan variant (0 = the heuristic). Bits 0-1: the lookahead's capacity is
+    // reduced by that many qubits; bit 2: the next tile is the candidate (four
+    // lookahead capacities and the most-targeted qubits) that runs the most
+    // ops; bit 3: register-tile programs give register slots to the most-used
+    // bits first. Every variant applies the same ops in a dependency-respecting
+    // order, so the amplitudes agree up to rounding.
+    uint32_t variant = 0;
 };
+
+constexpr uint32_t PLAN_VARIANTS = 16;
 
 // The heuristic plan: src/blocking.rs's extract / plan_swaps loop with the
 // lookahead's wanted set becoming the next stage's tile qubits.

@@ -486,6 +486,17 @@ struct Gen {
             wanted.push_back((uint8_t)b);
             return true;
         });
+        if (st.role_policy == 1) {
+            // Most-used first: the wanted bits by the number of pending ops
+            // that need them in this stage (ties keep the order of need).
+            std::vector<uint32_t> uses(st.local.size(), 0);
+            for (size_t i = 0; i < st.ops.size(); i++) {
+                if (done[i]) continue;
+                const int b = need(st.ops[i]);
+                if (b >= 0) uses[b]++;
+            }
+            std::stable_sort(wanted.begin(), wanted.end(), [&](uint8_t a, uint8_t b) { return uses[a] > uses[b]; });
+        }
         return wanted;
     }
 

@@ -27,8 +27,11 @@ EXPORTS = (
     "qvd_sv_apply_ops", "qvd_plan_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
     "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_compare", "qvd_sv_last_sample_seconds",
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
-    "qvd_batch_outputs",
+    "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
+    "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
+    "qvd_mlp_train", "qvd_planner_set_model",
 )
+PLAN_VARIANTS, PLAN_FEATURES = 16, 24
 
 
 class NativeError(RuntimeError):
@@ -54,6 +57,7 @@ class Options(C.Structure):
         ("tile_bits", C.c_int32),
         ("planner", C.c_uint32),
         ("flags", C.c_uint32),
+        ("variant", C.c_uint32),
     ]
 
 
@@ -76,6 +80,7 @@ class Stats(C.Structure):
         ("plan_seconds", C.c_double),
         ("stage_kernel_seconds", C.c_double),
         ("other_kernel_seconds", C.c_double),
+        ("plan_variant", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -122,6 +127,18 @@ def lib():
         L.qvd_batch_destroy.argtypes = [vp]
         L.qvd_batch_run.argtypes = [vp, P(d), P(d), P(Stats)]
         L.qvd_batch_outputs.argtypes = [vp, P(u64), P(u64)]
+        f = C.c_float
+        L.qvd_plan_features.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(f), u64, P(u64)]
+        L.qvd_sv_stage_seconds.argtypes = [vp, P(d), u64, P(u64)]
+        L.qvd_mlp_create.argtypes = [u32, u32, P(vp)]
+        L.qvd_mlp_destroy.argtypes = [vp]
+        L.qvd_mlp_num_params.argtypes = [vp, P(u64)]
+        L.qvd_mlp_set_params.argtypes = [vp, P(f), u64]
+        L.qvd_mlp_get_params.argtypes = [vp, P(f), u64]
+        L.qvd_mlp_forward.argtypes = [vp, P(f), u64, P(f)]
+        L.qvd_mlp_gradient.argtypes = [vp, P(f), P(f), u64, P(f), P(d)]
+        L.qvd_mlp_train.argtypes = [vp, P(f), P(f), u64, u32, u32, f, u64, P(d)]
+        L.qvd_planner_set_model.argtypes = [vp, P(f), P(f), f, f]
         for name in EXPORTS:
             getattr(L, name)  # fail early on a missing symbol
         _lib = L
@@ -195,11 +212,25 @@ class OpArray:
             self.arr[i] = Op(_p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size, _p(m.view(np.float64), C.c_double))
 
 
+def plan_features(num_qubits: int, ops, precision: str = "single", variant: int = 0,
+                  max_fused_qubits: int = 4) -> np.ndarray:
+    """Per register-tile stage features (rows x PLAN_FEATURES) of a plan variant."""
+    arr = ops if isinstance(ops, OpArray) else OpArray(ops)
+    opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0, variant)
+    prec = {"single": C64, "double": C128}[precision]
+    rows = C.c_uint64()
+    _check(lib().qvd_plan_features(num_qubits, prec, arr.arr, arr.n, C.byref(opt), None, 0, C.byref(rows)))
+    out = np.zeros((max(rows.value, 1), PLAN_FEATURES), dtype=np.float32)
+    _check(lib().qvd_plan_features(num_qubits, prec, arr.arr, arr.n, C.byref(opt), _p(out, C.c_float), rows.value,
+                                   C.byref(rows)))
+    return out[: rows.value]
+
+
 def plan_ops(num_qubits: int, ops, precision: str = "single", max_fused_qubits: int = 4, tile_bits: int | None = 0,
-             planner: int = PLANNER_HEURISTIC) -> dict:
+             planner: int = PLANNER_HEURISTIC, variant: int = 0) -> dict:
     """Plan statistics (stages, passes, fused ops) of apply_ops, on the host only."""
     arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-    opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0)
+    opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0, variant)
     st = Stats()
     _check(lib().qvd_plan_ops(num_qubits, {"single": C64, "double": C128}[precision], arr.arr, arr.n, C.byref(opt), C.byref(st)))
     return st.as_dict()
@@ -262,13 +293,26 @@ class StateVector:
         _check(lib().qvd_sv_apply(self.h, _p(m.view(np.float64), C.c_double), _p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size))
 
     def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC,
-                  timing: bool = False) -> dict:
-        """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default."""
+                  timing: bool = False, variant: int = 0) -> dict:
+        """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default.
+
+        planner PLANNER_LEARNED scores every plan variant with the installed
+        cost model (heuristic when none is installed); variant picks a
+        heuristic variant directly."""
         arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, FLAG_TIMING if timing else 0)
+        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, FLAG_TIMING if timing else 0,
+                      variant)
         st = Stats()
         _check(lib().qvd_sv_apply_ops(self.h, arr.arr, arr.n, C.byref(opt), C.byref(st)))
         return st.as_dict()
+
+    def stage_seconds(self) -> np.ndarray:
+        """Kernel time of each register-tile stage of the last timed apply_ops."""
+        n = C.c_uint64()
+        _check(lib().qvd_sv_stage_seconds(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.float64)
+        _check(lib().qvd_sv_stage_seconds(self.h, _p(out, C.c_double), n.value, C.byref(n)))
+        return out[: n.value]
 
     def norm_sqr(self) -> float:
         v = C.c_double()
@@ -335,7 +379,7 @@ class Batch:
         cs = (CircuitDesc * len(self.sizes))()
         for i, (n, arr) in enumerate(zip(self.sizes, self.arrays)):
             cs[i] = CircuitDesc(n, arr.arr, arr.n)
-        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0)
+        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0, 0)
         h = C.c_void_p()
         _check(lib().qvd_batch_create(cs, len(self.sizes), {"single": C64, "double": C128}[precision], C.byref(opt),
                                       C.byref(h)))
@@ -374,3 +418,71 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Mlp:
+    """The planner's cost model network on the device (qvd_mlp_*): inputs ->
+    hidden -> hidden -> 1 with ReLU, FP32, trained by the library's kernels."""
+
+    def __init__(self, inputs: int, hidden: int = 64):
+        self.inputs, self.hidden = inputs, hidden
+        h = C.c_void_p()
+        _check(lib().qvd_mlp_create(inputs, hidden, C.byref(h)))
+        self.h = h
+        n = C.c_uint64()
+        _check(lib().qvd_mlp_num_params(self.h, C.byref(n)))
+        self.num_params = n.value
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().qvd_mlp_set_params(self.h, _p(p, C.c_float), p.size))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, dtype=np.float32)
+        _check(lib().qvd_mlp_get_params(self.h, _p(p, C.c_float), p.size))
+        return p
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.inputs)
+        y = np.zeros(max(len(x), 1), dtype=np.float32)
+        _check(lib().qvd_mlp_forward(self.h, _p(x, C.c_float), len(x), _p(y, C.c_float)))
+        return y[: len(x)]
+
+    def gradient(self, x, t) -> tuple:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.inputs)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().qvd_mlp_gradient(self.h, _p(x, C.c_float), _p(t, C.c_float), len(x), _p(g, C.c_float),
+                                      C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs: int, batch: int = 256, lr: float = 1e-3, seed: int = 0) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.inputs)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().qvd_mlp_train(self.h, _p(x, C.c_float), _p(t, C.c_float), len(x), epochs, batch, lr, seed,
+                                   C.byref(loss)))
+        return loss.value
+
+    def install(self, mean, scale, y_mean: float, y_scale: float) -> None:
+        """Make this network the learned planner's cost model."""
+        mean = np.ascontiguousarray(mean, dtype=np.float32)
+        scale = np.ascontiguousarray(scale, dtype=np.float32)
+        _check(lib().qvd_planner_set_model(self.h, _p(mean, C.c_float), _p(scale, C.c_float), y_mean, y_scale))
+
+    def close(self) -> None:
+        if self.h:
+            lib().qvd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def clear_planner_model() -> None:
+    """Remove the learned planner's cost model (it then falls back to the heuristic)."""
+    _check(lib().qvd_planner_set_model(None, None, None, 0.0, 1.0))

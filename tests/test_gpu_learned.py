@@ -1,0 +1,130 @@
+"""The learned planner: the cost-model network's kernels against the CPU
+oracle (oracle/mlp.py), training, plan variants (same amplitudes, different
+stage partitions) and the planner's choice with and without a model."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp as ref
+from qevedo.simulator import StateVector, library, lower_gates
+from qevedo.simulator import _native, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def random_params(inputs, hidden, seed):
+    return learned.init_params(inputs, hidden, seed) + np.random.Generator(np.random.PCG64(seed)).normal(
+        0, 0.05, ref.num_params(inputs, hidden)).astype(np.float32)
+
+
+def test_forward_and_gradient_match_the_oracle():
+    inputs, hidden, rows = _native.PLAN_FEATURES, 64, 300
+    rng = np.random.Generator(np.random.PCG64(1))
+    p = random_params(inputs, hidden, 2)
+    x = rng.normal(0, 1, (rows, inputs)).astype(np.float32)
+    t = rng.normal(0, 1, rows).astype(np.float32)
+    net = _native.Mlp(inputs, hidden)
+    assert net.num_params == ref.num_params(inputs, hidden)
+    net.set_params(p)
+    assert np.array_equal(net.params(), p)
+    y = net.forward(x)
+    y_ref, _, _ = ref.forward(p, x, inputs, hidden)
+    assert np.max(np.abs(y - y_ref)) < 1e-4 * max(1.0, np.max(np.abs(y_ref)))
+    g, loss = net.gradient(x, t)
+    g_ref, loss_ref = ref.gradient(p, x, t, inputs, hidden)
+    assert abs(loss - loss_ref) < 1e-4 * max(1.0, loss_ref)
+    assert np.max(np.abs(g - g_ref)) < 1e-4 * max(1.0, np.max(np.abs(g_ref)))
+    net.close()
+
+
+def test_training_fits_a_smooth_function():
+    rng = np.random.Generator(np.random.PCG64(3))
+    inputs = 6
+    x = rng.normal(0, 1, (2048, inputs)).astype(np.float32)
+    t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2] - 0.3 * x[:, 3] ** 2).astype(np.float32)
+    net = _native.Mlp(inputs, 64)
+    net.set_params(learned.init_params(inputs, 64, 4))
+    before = float(np.mean((net.forward(x) - t) ** 2))
+    after = net.train(x, t, epochs=60, batch=128, lr=3e-3, seed=5)
+    assert after < 0.1 * before and after < 0.05
+    net.close()
+
+
+CIRCUITS = [
+    ("random", 18, lambda n: library.random_circuit(n, 12, 3)),
+    ("qft", 17, lambda n: library.qft(n)),
+    ("ising", 19, lambda n: library.kicked_ising(n, library.heavy_hex_edges(n), 4, 0.9)),
+]
+
+
+@pytest.mark.parametrize("name,n,make", CIRCUITS)
+def test_every_variant_gives_the_same_state(name, n, make):
+    ops = _native.OpArray(lower_gates(make(n)))
+    base = StateVector(n, "double")
+    base.apply_ops(ops)
+    ref64 = StateVector(n, "single")
+    ref64.apply_ops(ops)
+    for v in range(_native.PLAN_VARIANTS):
+        print(f"{name} {n}q variant {v}", flush=True)
+        s = StateVector(n, "double")
+        st = s.apply_ops(ops, variant=v)
+        assert st["plan_variant"] == v
+        e = s.compare(base)
+        assert e["rel_l2"] < 1e-12
+        s.close()
+        s = StateVector(n, "single")
+        s.apply_ops(ops, variant=v)
+        assert s.compare(base)["rel_l2"] < 1e-5
+        s.close()
+    base.close()
+    ref64.close()
+
+
+def test_learned_planner_falls_back_and_chooses_the_predicted_best():
+    n = 20
+    ops = _native.OpArray(lower_gates(library.random_circuit(n, 10, 9)))
+    _native.clear_planner_model()
+    s = StateVector(n, "single")
+    st = s.apply_ops(ops, planner=_native.PLANNER_LEARNED)
+    assert st["plan_variant"] == 0  # no model: the heuristic
+    # A model that predicts log time = sum of the features (all weights
+    # positive) prefers the variant with the fewest, smallest stages.
+    inputs, hidden = _native.PLAN_FEATURES, 64
+    net = _native.Mlp(inputs, hidden)
+    W1 = np.full((hidden, inputs), 0.01, np.float32)
+    p = np.concatenate([W1.ravel(), np.zeros(hidden), np.eye(hidden, dtype=np.float32).ravel(), np.zeros(hidden),
+                        np.full(hidden, 1.0 / hidden, np.float32), [0.0]]).astype(np.float32)
+    net.set_params(p)
+    net.install(np.zeros(inputs), np.ones(inputs), -25.0, 1.0)
+    scores = []
+    for v in range(_native.PLAN_VARIANTS):
+        f = _native.plan_features(n, ops, variant=v)
+        scores.append(float(np.sum(np.exp(-25.0 + 0.01 * f.sum(axis=1)))))
+    st = s.apply_ops(ops, planner=_native.PLANNER_LEARNED)
+    assert st["plan_variant"] == int(np.argmin(scores))
+    _native.clear_planner_model()
+    net.close()
+    s.close()
+
+
+@pytest.mark.skipif(not os.path.exists(learned.DEFAULT_MODEL), reason="no trained planner model in the tree")
+def test_trained_model_plan_is_not_slower():
+    model = learned.load()
+    net = learned.install(model)
+    n = 26
+    gates = library.kicked_ising(n, library.heavy_hex_edges(n), 8, 0.45)
+    ops = _native.OpArray(lower_gates(gates))
+    s = StateVector(n, "single")
+    times = {}
+    for planner in (_native.PLANNER_HEURISTIC, _native.PLANNER_LEARNED, _native.PLANNER_HEURISTIC,
+                    _native.PLANNER_LEARNED):
+        s.set_zero()
+        st = s.apply_ops(ops, planner=planner, timing=True)
+        times.setdefault(planner, []).append(st["stage_kernel_seconds"] + st["other_kernel_seconds"])
+    h, l = min(times[_native.PLANNER_HEURISTIC]), min(times[_native.PLANNER_LEARNED])
+    print(f"heuristic {h * 1e3:.2f} ms, learned {l * 1e3:.2f} ms", flush=True)
+    assert l <= 1.05 * h
+    _native.clear_planner_model()
+    net.close()
+    s.close()
