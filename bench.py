@@ -75,11 +75,11 @@ def run_batch(count: int) -> dict:
         sv.close()
         worst = max(worst, float(np.max(np.abs(z - out[i][0]))), float(np.max(np.abs(zz - out[i][1]))) if n > 1 else 0.0)
     single_s = time.perf_counter() - t
-    log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run ({stats['passes']} rounds, "
+    log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run ({stats['passes']} stage levels, "
         f"{stats['stages']} stage descriptors; build {build_s:.2f} s); one by one {single_s:.3f} s; "
         f"max |diff| {worst:.2e}")
     return {"circuits": count, "qubits": "12..22", "seconds": batch_s, "one_by_one_seconds": single_s,
-            "speedup": single_s / batch_s, "rounds": stats["passes"], "build_seconds": build_s,
+            "speedup": single_s / batch_s, "stage_levels": stats["passes"], "build_seconds": build_s,
             "max_abs_diff_vs_one_by_one": worst}
 
 

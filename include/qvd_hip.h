@@ -211,8 +211,8 @@ int qvd_planner_set_model(const qvd_mlp *mlp, const float *mean, const float *sc
 
 /* ---- Batches of circuits ------------------------------------------------
  * Many circuits, each on its own state, run together: the engine plans every
- * circuit into register-tile stages, then runs stage round r of all circuits
- * in one kernel launch (one descriptor per circuit). Reset, the rounds and
+ * circuit into register-tile stages, then runs stage level r (stage r of every
+ * circuit) in one kernel launch (one descriptor per circuit). Reset, the levels and
  * the expectation values are captured once in a HIP graph at creation and
  * replayed by every qvd_batch_run. States smaller than one register tile
  * (14 qubits in complex64, 13 in complex128) are widened with ancilla qubits
@@ -237,7 +237,7 @@ int qvd_batch_destroy(qvd_batch *batch);
 /* Runs every circuit from |0...0> and writes <Z_q> (num_qubits values per
  * circuit) to z and <Z_q Z_{q+1}> (num_qubits - 1 per circuit) to zz, both
  * concatenated in circuit order, FP64. stats: gates, kernels, passes (stage
- * rounds), stages (register-tile stage descriptors), gate_seconds (the graph
+ * levels), stages (register-tile stage descriptors), gate_seconds (the graph
  * replay, host clock). */
 int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats);
 /* Sizes of the z and zz arrays qvd_batch_run fills. */

@@ -94,8 +94,8 @@ struct qvd_sv {
 };
 
 // A batch: every circuit's state back to back in one allocation, the
-// register-tile programs of all stages, one launch table per stage round and
-// the HIP graph that replays reset, rounds and expectation values.
+// register-tile programs of all stages, one launch table per stage level and
+// the HIP graph that replays reset, levels and expectation values.
 struct qvd_batch {
     int precision = QVD_C64;
     int device = 0;
@@ -106,7 +106,7 @@ struct qvd_batch {
     uint64_t gates = 0, kernels = 0, stages = 0;
     DBuf amps, ins, coefs, ldsops, ints, descs, block_start, basis;
     DBuf edescs, seg_start, out_start, seg_sums, z, zz;
-    uint32_t rounds = 0;
+    uint32_t levels = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~qvd_batch() {
@@ -646,7 +646,7 @@ void count_plan(const std::vector<Step> &steps, qvd_stats &st) {
 }
 
 // Builds a batch: plans each circuit on max(n, L) qubits (L = the register
-// tile), lowers its stages, lays out the round launch tables and the
+// tile), lowers its stages, lays out the level launch tables and the
 // expectation descriptors, uploads them and captures the graph.
 template <typename T2>
 void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, const qvd_options &opt_in) {
@@ -707,16 +707,16 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
         off += uint64_t(1) << npad;
     }
     b->len = off;
-    // Round r: stage r of every circuit that has one; one workgroup per tile.
+    // Level r: stage r of every circuit that has one; one workgroup per tile.
     std::vector<VStageDesc> all_descs;
     std::vector<uint32_t> all_bs;
-    struct Round {
+    struct Level {
         uint32_t d0, nd, bs0;
         unsigned grid;
     };
-    std::vector<Round> rounds;
+    std::vector<Level> levels;
     for (uint32_t r = 0;; r++) {
-        Round rd{(uint32_t)all_descs.size(), 0, (uint32_t)all_bs.size(), 0};
+        Level rd{(uint32_t)all_descs.size(), 0, (uint32_t)all_bs.size(), 0};
         uint64_t blocks = 0;
         all_bs.push_back(0);
         for (uint32_t c = 0; c < count; c++) {
@@ -733,10 +733,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
             break;
         }
         rd.grid = (unsigned)blocks;
-        rounds.push_back(rd);
+        levels.push_back(rd);
         b->stages += rd.nd;
     }
-    b->rounds = (uint32_t)rounds.size();
+    b->levels = (uint32_t)levels.size();
     // Upload.
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
@@ -767,7 +767,7 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     auto enqueue = [&]() {
         launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
         QVD_HIP(hipGetLastError());
-        for (const Round &rd : rounds)
+        for (const Level &rd : levels)
             QVD_HIP(launch_vstage<T2>(s, b->descs.as<VStageDesc>() + rd.d0, rd.nd, b->block_start.as<uint32_t>() + rd.bs0,
                                       b->ins.as<VInstr>(), b->coefs.as<T2>(), b->ldsops.as<StageOp>(),
                                       b->ints.as<uint32_t>(), rd.grid, b->stream));
@@ -1256,7 +1256,7 @@ int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats) {
             qvd_stats st{};
             st.gates = batch->gates;
             st.kernels = batch->kernels;
-            st.passes = batch->rounds;
+            st.passes = batch->levels;
             st.stages = batch->stages;
             st.gate_seconds = t1 - t0;
             *stats = st;

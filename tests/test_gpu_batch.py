@@ -1,5 +1,5 @@
 """Batches of circuits (qvd_batch_*): many hardware-efficient-ansatz circuits
-of uneven sizes in one graph-captured sequence of stage rounds, checked
+of uneven sizes in one graph-captured sequence of stage levels, checked
 circuit by circuit against the oracle's FP64 state vector and against the
 same circuits run one by one on single states."""
 import numpy as np
