@@ -83,6 +83,41 @@ def run_batch(count: int) -> dict:
             "max_abs_diff_vs_one_by_one": worst}
 
 
+def run_precision_form(n: int, depth: int, seed: int) -> dict:
+    """Precision form: the random circuit fused to dense ops of at most 6
+    qubits, each its own HBM pass (no stages), so every 5- and 6-qubit fused
+    gate runs as a complex GEMM on FP32 matrix cores; the same plan in
+    complex128 (VALU FP64) is the reference the complex64 result is compared
+    with on the device."""
+    from qevedo.simulator import StateVector, library, lower_gates
+    from qevedo.simulator import _native
+
+    ops = _native.OpArray(lower_gates(library.random_circuit(n, depth, seed)))
+    sv = StateVector(n, "single")
+    sv.apply_ops(ops, 6, None)
+    sv.set_zero()
+    t = time.perf_counter()
+    st = sv.apply_ops(ops, 6, None, timing=True)
+    c64_s = time.perf_counter() - t
+    sv.set_zero()
+    st_v = sv.apply_ops(ops, 6, None, timing=True, dense_valu=True)
+    sv.set_zero()
+    sv.apply_ops(ops, 6, None)
+    ref = StateVector(n, "double")
+    t = time.perf_counter()
+    ref.apply_ops(ops, 6, None)
+    c128_s = time.perf_counter() - t
+    err = sv.compare(ref)
+    ref.close()
+    sv.close()
+    log(f"precision form {n} q: {st['kernels']} fused ops, complex64 {c64_s:.3f} s (kernels "
+        f"{st['other_kernel_seconds']:.3f} s; VALU dense kernels {st_v['other_kernel_seconds']:.3f} s), "
+        f"complex128 {c128_s:.3f} s, max-abs {err['max_abs']:.2e}, rel L2 {err['rel_l2']:.2e}")
+    return {"qubits": n, "depth": depth, "fused_ops": st["kernels"], "c64_seconds": c64_s,
+            "c64_kernel_seconds": st["other_kernel_seconds"], "c64_valu_kernel_seconds": st_v["other_kernel_seconds"],
+            "c128_seconds": c128_s, "max_abs": err["max_abs"], "rel_l2": err["rel_l2"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +134,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched hardware-efficient-ansatz workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned planner")
+    ap.add_argument("--precision-form", type=int, default=0, metavar="QUBITS",
+                    help="also run the precision form (fusion to 6-qubit GEMMs on matrix cores) at this size")
     ap.add_argument("--batch-circuits", type=int, default=512)
     ap.add_argument("--calibration-passes", type=int, default=0,
                     help="extra in-place scale-by-1 passes (k_scale: one read and one write of the state), "
@@ -203,6 +240,8 @@ def main() -> int:
 
     if not args.no_batch:
         result_extra["batch"] = run_batch(args.batch_circuits)
+    if args.precision_form:
+        result_extra["precision_form"] = run_precision_form(args.precision_form, args.depth, args.seed)
 
     cpu = None
     if not args.no_cpu:

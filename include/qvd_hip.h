@@ -78,6 +78,9 @@ typedef struct qvd_options {
 /* qvd_options.flags: time every kernel launch with HIP events on the
  * handle's stream and report the sums in qvd_stats. */
 #define QVD_FLAG_TIMING 1u
+/* qvd_options.flags: apply 5- and 6-qubit dense ops in complex64 with VALU
+ * FMAs instead of the FP32 matrix cores (for comparisons). */
+#define QVD_FLAG_DENSE_VALU 2u
 
 /* What a run did: simulator::Stats (src/simulator.rs:70-93). */
 typedef struct qvd_stats {

@@ -10,6 +10,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "dense_mfma.h"
 #include "kernels.h"
 #include "mlp.h"
 #include "plan.h"
@@ -153,8 +154,18 @@ template <typename T2> T2 to_t2(C64 v) {
 
 // One op as its own HBM pass: the dense kernel over the controlled subspace,
 // or the diagonal kernel.
+int g_num_cus = 0;
+int num_cus(int device) {
+    if (g_num_cus == 0) {
+        hipDeviceProp_t p;
+        QVD_HIP(hipGetDeviceProperties(&p, device));
+        g_num_cus = p.multiProcessorCount;
+    }
+    return g_num_cus;
+}
+
 template <typename T2>
-void apply_full(qvd_sv *sv, T2 *s, const Op &op) {
+void apply_full(qvd_sv *sv, T2 *s, const Op &op, bool valu_only = false) {
     const uint32_t k = (uint32_t)op.targets.size();
     if (k == 0) return;
     if (k > MAX_GATE_QUBITS)
@@ -194,6 +205,14 @@ void apply_full(qvd_sv *sv, T2 *s, const Op &op) {
     for (size_t i = 0; i < m.size(); i++) m[i] = to_t2<T2>(op.m.a[i]);
     DBuf dev(m.size() * sizeof(T2));
     QVD_HIP(hipMemcpyAsync(dev.p, m.data(), dev.bytes, hipMemcpyHostToDevice, sv->stream));
+    if constexpr (sizeof(T2) == sizeof(float2)) {
+        // 5- and 6-qubit gates in complex64: FP32 matrix cores.
+        if ((k == 5 || k == 6) && !valu_only) {
+            QVD_HIP(launch_dense_mfma(s, d, dev.as<float2>(), (unsigned)num_cus(sv->device), sv->stream));
+            sv->sync();
+            return;
+        }
+    }
     launch_dense<T2>(s, d, dev.as<T2>(), sv->stream);
     QVD_HIP(hipGetLastError());
     sv->sync();
@@ -276,15 +295,6 @@ void lower_stage(const Stage &st, uint32_t n, std::vector<StageOp> &ops, std::ve
     }
 }
 
-int g_num_cus = 0;
-int num_cus(int device) {
-    if (g_num_cus == 0) {
-        hipDeviceProp_t p;
-        QVD_HIP(hipGetDeviceProperties(&p, device));
-        g_num_cus = p.multiProcessorCount;
-    }
-    return g_num_cus;
-}
 
 // Events around kernel launches on the handle's stream (QVD_FLAG_TIMING).
 struct LaunchTimer {
@@ -320,7 +330,8 @@ struct LaunchTimer {
 };
 
 template <typename T2>
-void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, bool timing) {
+void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags) {
+    const bool timing = (flags & QVD_FLAG_TIMING) != 0;
     LaunchTimer timer;
     timer.on = timing;
     timer.stream = sv->stream;
@@ -408,7 +419,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             st.passes++;
         } else {
             timer.begin(timer.other);
-            apply_full<T2>(sv, s, step.op);
+            apply_full<T2>(sv, s, step.op, (flags & QVD_FLAG_DENSE_VALU) != 0);
             timer.end(timer.other);
             st.kernels++;
             st.passes++;
@@ -975,7 +986,7 @@ int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_
         st.plan_variant = variant;
         sv->bind();
         with_type(sv, [&](auto *s) {
-            run_plan(sv, s, steps, st, (opt.flags & QVD_FLAG_TIMING) != 0);
+            run_plan(sv, s, steps, st, opt.flags);
             return 0;
         });
         const double t2 = now();

@@ -150,26 +150,34 @@ __device__ __forceinline__ T2 sel(bool p, T2 a, T2 b) {
     return mk<T2>(p ? a.x : b.x, p ? a.y : b.y);
 }
 
-// Negates a when bit 0 of f is set (sign-bit xor).
-__device__ __forceinline__ float2 flip_sign(float2 a, uint32_t f) {
-    const uint32_t m = f << 31;
+// Flips the sign of both parts of a where bit 31 of m is set.
+__device__ __forceinline__ float2 xor_sign(float2 a, uint32_t m) {
     return make_float2(__uint_as_float(__float_as_uint(a.x) ^ m), __uint_as_float(__float_as_uint(a.y) ^ m));
 }
-__device__ __forceinline__ double2 flip_sign(double2 a, uint32_t f) {
-    const unsigned long long m = (unsigned long long)f << 63;
-    return make_double2(__longlong_as_double(__double_as_longlong(a.x) ^ m),
-                        __longlong_as_double(__double_as_longlong(a.y) ^ m));
+__device__ __forceinline__ double2 xor_sign(double2 a, uint32_t m) {
+    const unsigned long long w = (unsigned long long)m << 32;
+    return make_double2(__longlong_as_double(__double_as_longlong(a.x) ^ w),
+                        __longlong_as_double(__double_as_longlong(a.y) ^ w));
 }
 
 // A run of CZ-like gates: amplitude (thread, register j) changes sign when
 // the quadratic form sum x_p x_q over the run's pairs is odd. Pairs of two
 // register bits arrive as the per-register mask v.neg; every other pair is
-// (encoded bit, encoded bit) in the int array.
+// (encoded bit, encoded bit) in the int array. The parity of register j is
+// built as t[j] = t[j with its lowest set bit cleared] ^ M[that bit], one
+// exclusive-or per register, kept in bit 31.
 template <typename T2, int RB>
 __device__ __forceinline__ void signs(T2 (&x)[1 << RB], const QVD_CONST VInstr &v, const QVD_CONST uint32_t *ints,
                                       uint32_t tid, uint64_t base) {
+    const uint64_t neg = v.neg;
+    const uint32_t k = v.k;
+    if (k == 0) {
+#pragma unroll
+        for (int j = 0; j < (1 << RB); j++) x[j] = xor_sign(x[j], (uint32_t)((neg >> j) & 1) << 31);
+        return;
+    }
     uint32_t al = 0, gam = 0, gm = 0;
-    for (uint32_t i = 0; i < v.k; i++) {
+    for (uint32_t i = 0; i < k; i++) {
         const uint32_t a = ints[v.aux + 2 * i], b = ints[v.aux + 2 * i + 1];
         const uint32_t ka = a & VT_KIND, kb = b & VT_KIND, xa = a & 0xffffu, xb = b & 0xffffu;
         const uint32_t va = ka == VT_THREAD ? (tid >> xa) & 1 : (uint32_t)(base >> xa) & 1;
@@ -184,17 +192,12 @@ __device__ __forceinline__ void signs(T2 (&x)[1 << RB], const QVD_CONST VInstr &
         }
     }
     const uint32_t mask = gam ^ gm;
-    if (v.k == 0) {
+    uint32_t t[1 << RB];
+    t[0] = al << 31;
 #pragma unroll
-        for (int j = 0; j < (1 << RB); j++)
-            if ((v.neg >> j) & 1) x[j] = flip_sign(x[j], 1);
-        return;
-    }
+    for (int j = 1; j < (1 << RB); j++) t[j] = t[j & (j - 1)] ^ (((mask >> __builtin_ctz(j)) & 1) << 31);
 #pragma unroll
-    for (int j = 0; j < (1 << RB); j++) {
-        const uint32_t f = al ^ (uint32_t)((v.neg >> j) & 1) ^ __popc((uint32_t)j & mask);
-        x[j] = flip_sign(x[j], f & 1);
-    }
+    for (int j = 0; j < (1 << RB); j++) x[j] = xor_sign(x[j], t[j] ^ ((uint32_t)((neg >> j) & 1) << 31));
 }
 
 // a + i^K b: adds only.
@@ -741,10 +744,11 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
                          const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
                          hipStream_t st) {
     const size_t lds = vstage_lds_bytes<T2>();
-    auto k = sizeof(T2) == 8 ? (void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
-                                         const T2 *, const StageOp *, const uint32_t *))k_vstage<T2, 3, 5, 4>
-                             : (void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
-                                         const T2 *, const StageOp *, const uint32_t *))k_vstage<T2, 3, 4, 3>;
+    using Kernel = void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *, const T2 *,
+                            const StageOp *, const uint32_t *);
+    Kernel k;
+    if constexpr (sizeof(T2) == 8) k = k_vstage<T2, 3, 5, 4>;
+    else k = k_vstage<T2, 3, 4, 3>;
     // The LDS size attribute is set once per device, before any launch a
     // caller may capture into a graph.
     static std::atomic<uint64_t> attr_set{0};

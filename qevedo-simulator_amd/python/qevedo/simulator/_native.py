@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("QVD_LIB", os.path.join(_PKG_ROOT, "lib", "libqvd_hip.
 QVD_OK, QVD_EINVAL, QVD_ENOMEM, QVD_EHIP, QVD_EUNSUPPORTED = 0, -1, -2, -3, -4
 C64, C128 = 0, 1
 PLANNER_HEURISTIC, PLANNER_LEARNED = 0, 1
-FLAG_TIMING = 1
+FLAG_TIMING, FLAG_DENSE_VALU = 1, 2
 
 # Every symbol include/qvd_hip.h declares.
 EXPORTS = (
@@ -293,15 +293,15 @@ class StateVector:
         _check(lib().qvd_sv_apply(self.h, _p(m.view(np.float64), C.c_double), _p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size))
 
     def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC,
-                  timing: bool = False, variant: int = 0) -> dict:
+                  timing: bool = False, variant: int = 0, dense_valu: bool = False) -> dict:
         """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default.
 
         planner PLANNER_LEARNED scores every plan variant with the installed
         cost model (heuristic when none is installed); variant picks a
         heuristic variant directly."""
         arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, FLAG_TIMING if timing else 0,
-                      variant)
+        flags = (FLAG_TIMING if timing else 0) | (FLAG_DENSE_VALU if dense_valu else 0)
+        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, flags, variant)
         st = Stats()
         _check(lib().qvd_sv_apply_ops(self.h, arr.arr, arr.n, C.byref(opt), C.byref(st)))
         return st.as_dict()
