@@ -41,6 +41,17 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(n: int):
+    """HBM bytes per stage launch of this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy pass
+    of known size), or None when no such measurement exists for this size."""
+    path = os.path.join(ROOT, "profiles", "k_vstage_random30_c64_traffic.json")
+    if n != 30 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["hbm_bytes_per_launch"]
+
+
 def run_batch(count: int) -> dict:
     """Batching workload: `count` hardware-efficient-ansatz circuits (6 layers
     of RY, RZ on every qubit plus a CZ ladder; qubit counts uniform in 12..22,
@@ -278,7 +289,7 @@ def main() -> int:
                    "max_fused_qubits": args.fusion, "tile_bits": args.tile_bits or "register tile, 2^14 amplitudes",
                    "planner": "learned" if learned_s is not None else "heuristic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n)},
         "cpu_baseline": cpu,
         "error": error,
         "plans": {"heuristic": sec, "learned": learned_s},
