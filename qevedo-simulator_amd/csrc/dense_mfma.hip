@@ -6,9 +6,10 @@
 // U = Ur + i Ui and a column a = ar + i ai, the output is
 //     [or; oi] = R [ar; ai],   R = [[Ur, -Ui], [Ui, Ur]]   (2D x 2D real),
 // so a block of NC columns is the real product (2D x 2D) x (2D x NC).
-// A workgroup of 4 waves keeps R in LDS for all of its column blocks, stages
-// each block's amplitudes through LDS (coalesced loads and stores), and each
-// wave computes the 2D x 16 output of its 16 columns.
+// A workgroup of 4 waves stages each block's amplitudes through LDS
+// (coalesced loads and stores, the next block's loads in flight during the
+// products); each wave keeps its 2D/4 rows of R in registers for the whole
+// launch and computes those rows of the output for all NC columns.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,78 +37,133 @@ __device__ __forceinline__ uint64_t spread(uint64_t x, const uint32_t *pos, uint
 template <int K>
 __global__ __launch_bounds__(64 * WAVES) void k_dense_mfma(float2 *__restrict__ s, const GateDesc d,
                                                            const float2 *__restrict__ mat) {
-    constexpr int D = 1 << K, M = 2 * D;  // R is M x M
+    constexpr int D = 1 << K, M = 2 * D;       // R is M x M
+    constexpr int MPW = M / 16 / WAVES;        // 16-row tiles of R per wave
+    constexpr int KS = M / 4;                  // k steps of 4
+    constexpr int PER = D * NC / (64 * WAVES); // amplitudes staged per thread
     extern __shared__ float lds[];
-    float *R = lds;              // M x M, row-major
-    float *Br = R + M * M;       // D x PAD: real parts, row j = target index, column = block column
-    float *Bi = Br + D * PAD;    // D x PAD: imaginary parts
+    float *Br = lds;                                 // D x PAD: real parts (row j, column c)
+    float *Bi = Br + D * PAD;                        // D x PAD: imaginary parts
+    uint64_t *colbase = (uint64_t *)(Bi + D * PAD);  // 2 x NC: first amplitude of each column (double-buffered)
+    uint64_t *offs = colbase + 2 * NC;               // D: offset of target index j
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    for (uint32_t e = tid; e < (uint32_t)(M * M); e += 64 * WAVES) {
-        const uint32_t r = e / M, k = e % M;
-        const uint32_t rr = r % D, kk = k % D;
-        const float2 u = mat[rr * D + kk];
-        const bool rim = r >= (uint32_t)D, kim = k >= (uint32_t)D;
-        R[e] = rim == kim ? u.x : (rim ? u.y : -u.y);
-    }
-
-    // Load order: target index fastest when the lowest target is qubit 0
-    // (then a column's amplitudes are contiguous), else column fastest.
-    const bool jfast = d.offsets[1] == 1;
     const uint64_t blocks = (d.iterations + NC - 1) / NC;
-    for (uint64_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
-        const uint64_t c0 = blk * NC;
-        __syncthreads();  // R written / previous block's output read
-        for (uint32_t e = tid; e < (uint32_t)(D * NC); e += 64 * WAVES) {
-            const uint32_t j = jfast ? e % D : e / NC, c = jfast ? e / D : e % NC;
-            float2 a = make_float2(0.0f, 0.0f);
-            if (c0 + c < d.iterations) a = s[(spread(c0 + c, d.ins, d.nins) | d.ctrl_val) + d.offsets[j]];
-            Br[j * PAD + c] = a.x;
-            Bi[j * PAD + c] = a.y;
+    // Target index fastest when the lowest target is qubit 0 (a column's
+    // amplitudes are then contiguous), else column fastest.
+    const bool jfast = d.offsets[1] == 1;
+    auto elem = [&](uint32_t e, uint32_t &j, uint32_t &c) {
+        j = jfast ? e % D : e / NC;
+        c = jfast ? e / D : e % NC;
+    };
+    auto bases = [&](uint64_t blk, uint64_t *out) {
+        if (tid < (uint32_t)NC) {
+            const uint64_t col = blk * NC + tid;
+            out[tid] = col < d.iterations ? (spread(col, d.ins, d.nins) | d.ctrl_val) : ~uint64_t(0);
         }
-        __syncthreads();
+    };
 
-        // acc[mt] = rows mt*16 .. mt*16+15 of this wave's 16 columns.
-        floatx4 acc[M / 16];
+    // This wave's rows of R stay in registers for the whole launch: lane l
+    // holds R[row][4 ks + l / 16] of row = 16 mt + l % 16.
+    float ra[MPW][KS];
 #pragma unroll
-        for (int mt = 0; mt < M / 16; mt++) acc[mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        const uint32_t col = wave * 16 + (lane & 15), ksub = lane >> 4;
-#pragma unroll 4
-        for (int ks = 0; ks < M / 4; ks++) {
-            const uint32_t k = ks * 4 + ksub;
-            const float b = k < (uint32_t)D ? Br[k * PAD + col] : Bi[(k - D) * PAD + col];
+    for (int i = 0; i < MPW; i++) {
+        const uint32_t row = (wave * MPW + i) * 16 + (lane & 15);
+        const uint32_t rr = row % D;
+        const bool rim = row >= (uint32_t)D;
 #pragma unroll
-            for (int mt = 0; mt < M / 16; mt++) {
-                const float a = R[(mt * 16 + (lane & 15)) * M + k];
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[mt], 0, 0, 0);
+        for (int ks = 0; ks < KS; ks++) {
+            const uint32_t k = ks * 4 + (lane >> 4), kk = k % D;
+            const bool kim = k >= (uint32_t)D;
+            const float2 u = mat[rr * D + kk];
+            ra[i][ks] = rim == kim ? u.x : (rim ? u.y : -u.y);
+        }
+    }
+    if (tid < (uint32_t)D) offs[tid] = d.offsets[tid];
+    uint64_t blk = blockIdx.x;
+    if (blk >= blocks) return;
+    uint32_t cur = 0;
+    bases(blk, colbase);
+    __syncthreads();
+    float2 pre[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        uint32_t j, c;
+        elem(tid + i * 64 * WAVES, j, c);
+        const uint64_t b = colbase[c];
+        pre[i] = b != ~uint64_t(0) ? s[b + offs[j]] : make_float2(0.0f, 0.0f);
+    }
+    for (;;) {
+        __syncthreads();  // the previous block's output has been stored
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            uint32_t j, c;
+            elem(tid + i * 64 * WAVES, j, c);
+            Br[j * PAD + c] = pre[i].x;
+            Bi[j * PAD + c] = pre[i].y;
+        }
+        const uint64_t next = blk + gridDim.x;
+        if (next < blocks) bases(next, colbase + (cur ^ 1) * NC);
+        __syncthreads();
+        // The next block's loads fly while the matrix cores work.
+        if (next < blocks) {
+            const uint64_t *nb = colbase + (cur ^ 1) * NC;
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                uint32_t j, c;
+                elem(tid + i * 64 * WAVES, j, c);
+                const uint64_t b = nb[c];
+                pre[i] = b != ~uint64_t(0) ? s[b + offs[j]] : make_float2(0.0f, 0.0f);
             }
+        }
+        floatx4 acc[MPW][NC / 16];
+#pragma unroll
+        for (int i = 0; i < MPW; i++)
+#pragma unroll
+            for (int g = 0; g < NC / 16; g++) acc[i][g] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const uint32_t k = ks * 4 + (lane >> 4);
+            const float *row = k < (uint32_t)D ? Br + k * PAD : Bi + (k - D) * PAD;
+            float b[NC / 16];
+#pragma unroll
+            for (int g = 0; g < NC / 16; g++) b[g] = row[g * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < MPW; i++)
+#pragma unroll
+                for (int g = 0; g < NC / 16; g++)
+                    acc[i][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ra[i][ks], b[g], acc[i][g], 0, 0, 0);
         }
         __syncthreads();  // every wave has read Br / Bi
-        // Accumulator element i of lane l: row 4 * (l / 16) + i, column l % 16.
+        // Accumulator element e of lane l: row 4 * (l / 16) + e, column l % 16.
 #pragma unroll
-        for (int mt = 0; mt < M / 16; mt++) {
+        for (int i = 0; i < MPW; i++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint32_t row = mt * 16 + 4 * (lane >> 4) + i;
-                const uint32_t c = wave * 16 + (lane & 15);
-                if (row < (uint32_t)D) Br[row * PAD + c] = acc[mt][i];
-                else Bi[(row - D) * PAD + c] = acc[mt][i];
-            }
-        }
+            for (int g = 0; g < NC / 16; g++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint32_t row = (wave * MPW + i) * 16 + 4 * (lane >> 4) + e;
+                    const uint32_t c = g * 16 + (lane & 15);
+                    if (row < (uint32_t)D) Br[row * PAD + c] = acc[i][g][e];
+                    else Bi[(row - D) * PAD + c] = acc[i][g][e];
+                }
         __syncthreads();
+        const uint64_t *cb = colbase + cur * NC;
         for (uint32_t e = tid; e < (uint32_t)(D * NC); e += 64 * WAVES) {
-            const uint32_t j = jfast ? e % D : e / NC, c = jfast ? e / D : e % NC;
-            if (c0 + c < d.iterations)
-                s[(spread(c0 + c, d.ins, d.nins) | d.ctrl_val) + d.offsets[j]] =
-                    make_float2(Br[j * PAD + c], Bi[j * PAD + c]);
+            uint32_t j, c;
+            elem(e, j, c);
+            const uint64_t b = cb[c];
+            if (b != ~uint64_t(0)) s[b + offs[j]] = make_float2(Br[j * PAD + c], Bi[j * PAD + c]);
         }
+        if (next >= blocks) break;
+        blk = next;
+        cur ^= 1;
     }
 }
 
 template <int K>
 hipError_t launch_k(float2 *s, const GateDesc &d, const float2 *mat, unsigned grid, hipStream_t st) {
-    constexpr int D = 1 << K, M = 2 * D;
-    const size_t lds = (size_t)(M * M + 2 * D * PAD) * sizeof(float);
+    constexpr int D = 1 << K;
+    const size_t lds = (size_t)(2 * D * PAD) * sizeof(float) + (2 * NC + D) * sizeof(uint64_t);
     static std::atomic<uint64_t> attr_set{0};
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -126,7 +182,7 @@ hipError_t launch_k(float2 *s, const GateDesc &d, const float2 *mat, unsigned gr
 
 hipError_t launch_dense_mfma(float2 *s, const GateDesc &d, const float2 *mat, unsigned cus, hipStream_t st) {
     const uint64_t blocks = (d.iterations + NC - 1) / NC;
-    const unsigned grid = (unsigned)std::min<uint64_t>(blocks, (uint64_t)cus);
+    const unsigned grid = (unsigned)std::min<uint64_t>(blocks, (uint64_t)cus * 3);
     if (d.k == 5) return launch_k<5>(s, d, mat, grid, st);
     if (d.k == 6) return launch_k<6>(s, d, mat, grid, st);
     return hipErrorInvalidValue;

@@ -8,6 +8,8 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <thread>
+#include <unordered_map>
 
 #include "common.h"
 #include "dense_mfma.h"
@@ -556,8 +558,50 @@ struct PlannerModel {
     std::unique_ptr<Mlp> net;
     std::vector<float> mean, scale;
     float ymean = 0, yscale = 1;
+    // The variant chosen for each op sequence seen since the model was
+    // installed (hash of the state size, precision, options and ops).
+    std::unordered_map<uint64_t, uint32_t> chosen;
 };
 PlannerModel g_model;
+
+uint64_t fnv(uint64_t h, const void *p, size_t bytes) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < bytes; i++) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+uint64_t ops_hash(uint32_t n, int precision, const qvd_options &opt, const qvd_op *ops, uint64_t num_ops) {
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, &n, sizeof n);
+    h = fnv(h, &precision, sizeof precision);
+    h = fnv(h, &opt.max_fused_qubits, sizeof opt.max_fused_qubits);
+    h = fnv(h, &opt.tile_bits, sizeof opt.tile_bits);
+    for (uint64_t i = 0; i < num_ops; i++) {
+        const qvd_op &o = ops[i];
+        h = fnv(h, &o.num_targets, sizeof o.num_targets);
+        h = fnv(h, o.targets, o.num_targets * sizeof(uint32_t));
+        h = fnv(h, &o.num_controls, sizeof o.num_controls);
+        if (o.num_controls) {
+            h = fnv(h, o.controls, o.num_controls * sizeof(uint32_t));
+            h = fnv(h, o.control_values, o.num_controls);
+        }
+        h = fnv(h, o.matrix, (size_t(2) << (2 * o.num_targets)) * sizeof(double));
+    }
+    return h;
+}
+
+// A plan's identity for de-duplicating variants: its steps' tile qubits,
+// op counts and role policies.
+std::vector<uint32_t> plan_signature(const std::vector<Step> &steps) {
+    std::vector<uint32_t> sig;
+    for (const Step &st : steps) {
+        sig.push_back(st.is_stage ? (uint32_t)st.stage.ops.size() : 0xffffffffu);
+        if (!st.is_stage) continue;
+        sig.push_back(st.stage.role_policy);
+        sig.insert(sig.end(), st.stage.local.begin(), st.stage.local.end());
+    }
+    return sig;
+}
 
 // Validates the ops and plans them for a state of n qubits in `precision`:
 // the heuristic (options.variant picks a variant of it), or, with
@@ -603,19 +647,65 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
         if (chosen) *chosen = p.variant;
         return split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
     }
-    // Learned: every variant, scored by the cost model in one batched
+    // Learned: a choice made for the same ops before is reused.
+    const uint64_t key = ops_hash(n, precision, opt, ops, num_ops);
+    auto hit = g_model.chosen.find(key);
+    if (hit != g_model.chosen.end()) {
+        p.variant = hit->second;
+        if (chosen) *chosen = p.variant;
+        return split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
+    }
+    // Otherwise every variant is planned (in parallel; identical plans are
+    // scored once) and all their stages are scored by the cost model in one
     // forward pass on the device.
     const uint32_t F = QVD_PLAN_FEATURES;
     std::vector<std::vector<Step>> plans(PLAN_VARIANTS);
+    std::vector<std::vector<float>> feats(PLAN_VARIANTS);
+    std::vector<char> dup(PLAN_VARIANTS, 0);
+    {
+        std::vector<std::thread> workers;
+        std::vector<std::exception_ptr> errors(PLAN_VARIANTS);
+        for (uint32_t var = 0; var < PLAN_VARIANTS; var++)
+            workers.emplace_back([&, var] {
+                try {
+                    PlanParams pv = p;
+                    pv.variant = var;
+                    plans[var] = split_for_coefficients(plan_heuristic(v, pv), n, precision, vg);
+                } catch (...) {
+                    errors[var] = std::current_exception();
+                }
+            });
+        for (auto &w : workers) w.join();
+        for (auto &e : errors)
+            if (e) std::rethrow_exception(e);
+        std::vector<std::vector<uint32_t>> sigs(PLAN_VARIANTS);
+        for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
+            sigs[var] = plan_signature(plans[var]);
+            for (uint32_t u = 0; u < var && !dup[var]; u++)
+                if (!dup[u] && sigs[u] == sigs[var]) dup[var] = 1;
+        }
+        workers.clear();
+        for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
+            if (dup[var]) continue;
+            workers.emplace_back([&, var] {
+                try {
+                    feats[var] = plan_features(plans[var], n, precision);
+                } catch (...) {
+                    errors[var] = std::current_exception();
+                }
+            });
+        }
+        for (auto &w : workers) w.join();
+        for (auto &e : errors)
+            if (e) std::rethrow_exception(e);
+    }
     std::vector<float> rows;
     std::vector<size_t> first(PLAN_VARIANTS + 1, 0);
     std::vector<double> fixed(PLAN_VARIANTS, 0.0);
     const double amps = std::ldexp(1.0, (int)n);
     const double elem = precision == QVD_C64 ? 8.0 : 16.0;
     for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
-        p.variant = var;
-        plans[var] = split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
-        std::vector<float> r = plan_features(plans[var], n, precision);
+        std::vector<float> &r = feats[var];
         for (size_t k = 0; k < r.size(); k++) {
             const size_t j = k % F;
             r[k] = (r[k] - g_model.mean[j]) / g_model.scale[j];
@@ -632,6 +722,7 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
     uint32_t best = 0;
     double best_t = 0;
     for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
+        if (dup[var]) continue;
         double t = fixed[var];
         for (size_t k = first[var]; k < first[var + 1]; k++)
             t += std::exp(g_model.ymean + g_model.yscale * pred[k]) * amps;
@@ -640,6 +731,8 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
             best_t = t;
         }
     }
+    if (g_model.chosen.size() >= 256) g_model.chosen.clear();
+    g_model.chosen[key] = best;
     if (chosen) *chosen = best;
     return std::move(plans[best]);
 }
@@ -1371,6 +1464,7 @@ int qvd_planner_set_model(const qvd_mlp *mlp, const float *mean, const float *sc
         std::lock_guard<std::mutex> lock(g_model.mu);
         if (!mlp) {
             g_model.net.reset();
+            g_model.chosen.clear();
             return;
         }
         require(mean != nullptr && scale != nullptr, "null normalisation");
@@ -1382,6 +1476,7 @@ int qvd_planner_set_model(const qvd_mlp *mlp, const float *mean, const float *sc
         std::unique_ptr<Mlp> net(new Mlp(mlp->net.shape().inputs, mlp->net.shape().hidden));
         net->set_params(p.data());
         g_model.net = std::move(net);
+        g_model.chosen.clear();
         g_model.mean.assign(mean, mean + QVD_PLAN_FEATURES);
         g_model.scale.assign(scale, scale + QVD_PLAN_FEATURES);
         g_model.ymean = y_mean;
