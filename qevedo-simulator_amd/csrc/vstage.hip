@@ -563,7 +563,11 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         pF[b] = uint64_t(1) << d.local_pos[d.rmapF[b]];
     }
 
-    T2 x[R], y[R];
+    // With 4 waves per SIMD (16-wave workgroups) the other waves cover a
+    // tile's load latency; with 2 the next tile is prefetched into a second
+    // register set.
+    constexpr bool PREFETCH = WB <= 3;
+    T2 x[R], y[PREFETCH ? R : 1];
     auto tile_base = [&](uint64_t t) {
         uint64_t base = 0;
         for (uint32_t j = 0; j < d.nglob; j++) base |= ((t >> j) & 1) << d.glob_pos[j];
@@ -582,13 +586,20 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     };
 
     uint64_t t = bl;
-    if (t < d.num_tiles) load(y, tile_base(t));
+    if constexpr (PREFETCH)
+        if (t < d.num_tiles) load(y, tile_base(t));
     for (; t < d.num_tiles; t += nb) {
         const uint64_t base = tile_base(t);
 #pragma unroll
-        for (int j = 0; j < R; j++) x[j] = y[j];
-        const uint64_t tn = t + nb;
-        if (tn < d.num_tiles) load(y, tile_base(tn));
+        for (int j = 0; j < R; j++) {
+            if constexpr (PREFETCH) x[j] = y[j];
+        }
+        if constexpr (PREFETCH) {
+            const uint64_t tn = t + nb;
+            if (tn < d.num_tiles) load(y, tile_base(tn));
+        } else {
+            load(x, base);
+        }
 
         // Current role map: the thread's part of the tile index (fixed lane
         // bits, flexible lane bits, wave bits) and the register bits' tile
@@ -730,7 +741,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
 
 template <typename T2>
 VGeom vstage_geom() {
-    return sizeof(T2) == 8 ? VGeom{3, 5, 4} : VGeom{3, 4, 3};
+    return sizeof(T2) == 8 ? VGeom{4, 4, 4} : VGeom{3, 4, 3};
 }
 
 template <typename T2>
@@ -747,7 +758,7 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
     using Kernel = void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *, const T2 *,
                             const StageOp *, const uint32_t *);
     Kernel k;
-    if constexpr (sizeof(T2) == 8) k = k_vstage<T2, 3, 5, 4>;
+    if constexpr (sizeof(T2) == 8) k = k_vstage<T2, 4, 4, 4>;
     else k = k_vstage<T2, 3, 4, 3>;
     // The LDS size attribute is set once per device, before any launch a
     // caller may capture into a graph.
@@ -761,7 +772,7 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
         if (e != hipSuccess) return e;
         attr_set.fetch_or(bit);
     }
-    k<<<grid, 64 << 3, lds, st>>>(s, descs, ndesc, block_start, prog, coefs, ldsops, ints);
+    k<<<grid, 64 << vstage_geom<T2>().w_bits, lds, st>>>(s, descs, ndesc, block_start, prog, coefs, ldsops, ints);
     return hipGetLastError();
 }
 
