@@ -102,6 +102,13 @@ const char *qvd_last_error(void);
  * the architecture of device 0 (e.g. "gfx950"). */
 int qvd_device_info(int *num_devices, char *arch, uint32_t arch_capacity);
 
+/* state_vector_fits (src/simulator.rs:254-259): *fits = 1 when a state of
+ * num_qubits amplitudes in `precision` takes at most 80 % of the current
+ * device's HBM (the reference asks the same of host memory). *state_bytes and
+ * *device_bytes report both sides of the test. */
+int qvd_state_vector_fits(uint32_t num_qubits, int precision, int *fits, uint64_t *state_bytes,
+                          uint64_t *device_bytes);
+
 /* StateVector::zero (src/state.rs:39-49): |0...0> in HBM. */
 int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out);
 int qvd_sv_destroy(qvd_sv *sv);

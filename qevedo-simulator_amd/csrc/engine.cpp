@@ -921,6 +921,23 @@ int qvd_device_info(int *num_devices, char *arch, uint32_t arch_capacity) {
     });
 }
 
+int qvd_state_vector_fits(uint32_t num_qubits, int precision, int *fits, uint64_t *state_bytes,
+                          uint64_t *device_bytes) {
+    return guard([&] {
+        require(fits != nullptr, "null output");
+        require(precision == QVD_C64 || precision == QVD_C128, "precision must be QVD_C64 or QVD_C128");
+        require(num_qubits <= 63, "num_qubits out of range");
+        int dev = 0;
+        QVD_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        QVD_HIP(hipGetDeviceProperties(&p, dev));
+        const double bytes = std::ldexp(1.0, (int)num_qubits) * (precision == QVD_C64 ? 8.0 : 16.0);
+        *fits = bytes <= 0.8 * (double)p.totalGlobalMem ? 1 : 0;
+        if (state_bytes) *state_bytes = num_qubits < 60 ? (uint64_t(1) << num_qubits) * (precision == QVD_C64 ? 8 : 16) : ~uint64_t(0);
+        if (device_bytes) *device_bytes = (uint64_t)p.totalGlobalMem;
+    });
+}
+
 int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out) {
     return guard([&] {
         require(out != nullptr, "null output handle");

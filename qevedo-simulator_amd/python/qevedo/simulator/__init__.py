@@ -21,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import library
+from . import _native, library
 from ._native import (Batch, NativeError, StateVector, device_info, gate_lower, gate_matrix, lower_gates, plan_ops)
 from .circuit import Circuit
 
@@ -70,10 +70,16 @@ def _backend(name: str) -> str:
 
 
 def choose_backend(circuit, precision: str = "single") -> str:
-    """The state vector when it fits in HBM (simulator.rs:254-259, 304-323)."""
+    """choose_backend (simulator.rs:304-323) for this engine: the state vector
+    when it takes at most 80 % of the device's HBM (state_vector_fits,
+    simulator.rs:254-259). The reference's other backends (stabilizer,
+    near-Clifford, matrix product state) are not part of it, so a state that
+    does not fit is an error instead of a matrix product state."""
     c = _circuit(circuit)
-    if c.num_qubits > 34:
-        raise ValueError("a state vector of more than 34 qubits does not fit in memory")
+    fits, state_bytes, device_bytes = _native.state_vector_fits(c.num_qubits, precision)
+    if not fits:
+        raise ValueError(f"a {c.num_qubits}-qubit {precision} state vector ({state_bytes} bytes) does not fit in 80 % "
+                         f"of the device's {device_bytes} bytes of memory")
     return "statevector"
 
 
@@ -118,6 +124,7 @@ def run(circuit, shots: int = 1024, *, backend: str = "auto", seed: int | None =
     _backend(backend)
     if precision not in ("single", "double"):
         raise ValueError('precision must be "single" or "double"')
+    choose_backend(c, precision)
     rng = np.random.Generator(np.random.PCG64(seed))
     ins = c._ins
     split = _terminal_split(ins)

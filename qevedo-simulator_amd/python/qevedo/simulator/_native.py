@@ -29,7 +29,7 @@ EXPORTS = (
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
     "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
-    "qvd_mlp_train", "qvd_planner_set_model",
+    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits",
 )
 PLAN_VARIANTS, PLAN_FEATURES = 16, 24
 
@@ -101,6 +101,7 @@ def lib():
         u32, u64, i32, d, vp = C.c_uint32, C.c_uint64, C.c_int32, C.c_double, C.c_void_p
         L.qvd_last_error.restype = C.c_char_p
         L.qvd_device_info.argtypes = [P(C.c_int), C.c_char_p, u32]
+        L.qvd_state_vector_fits.argtypes = [u32, C.c_int, P(C.c_int), P(u64), P(u64)]
         L.qvd_sv_create.argtypes = [u32, C.c_int, P(vp)]
         L.qvd_sv_destroy.argtypes = [vp]
         L.qvd_sv_num_qubits.argtypes = [vp, P(u32)]
@@ -159,6 +160,17 @@ def device_info() -> tuple:
     arch = C.create_string_buffer(64)
     _check(lib().qvd_device_info(C.byref(n), arch, 64))
     return n.value, arch.value.decode()
+
+
+def state_vector_fits(num_qubits: int, precision: str = "single") -> tuple:
+    """(fits, state bytes, device bytes): whether the state takes at most 80 %
+    of the current device's HBM (simulator.rs:254-259)."""
+    prec = {"single": C64, "double": C128}.get(precision)
+    if prec is None:
+        raise ValueError('precision must be "single" or "double"')
+    fits, sb, db = C.c_int(), C.c_uint64(), C.c_uint64()
+    _check(lib().qvd_state_vector_fits(num_qubits, prec, C.byref(fits), C.byref(sb), C.byref(db)))
+    return bool(fits.value), sb.value, db.value
 
 
 def gate_matrix(name: str, params=()) -> np.ndarray:

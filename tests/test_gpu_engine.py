@@ -359,3 +359,25 @@ def test_errors_are_status_codes():
     with pytest.raises(NativeError) as e:
         StateVector(35)
     assert e.value.code == _native.QVD_EINVAL
+
+
+def test_state_vector_fits_device_memory():
+    """state_vector_fits (simulator.rs:254-259) against the device's HBM: a
+    state fits when it takes at most 80 % of it, and run() refuses one that
+    does not instead of falling back to another backend."""
+    _note("fits")
+    from qevedo.simulator import Circuit, choose_backend, run
+    fits, sb, db = _native.state_vector_fits(30, "single")
+    assert fits and sb == 8 << 30 and db > sb
+    for n in range(20, 40):
+        for prec, b in (("single", 8), ("double", 16)):
+            fits, sb, db2 = _native.state_vector_fits(n, prec)
+            assert db2 == db and sb == b << n
+            assert fits == (sb <= 0.8 * db)
+    big = Circuit(40)
+    big.h(0)
+    with pytest.raises(ValueError, match="does not fit"):
+        choose_backend(big)
+    with pytest.raises(ValueError, match="does not fit"):
+        run(big, shots=1)
+    assert choose_backend(Circuit(12)) == "statevector"
