@@ -295,10 +295,11 @@ struct Gen {
             // [[u0, u1], [u2, u3]] = diag(u0, u3) [[1, u1/u0], [u2/u3, 1]]
             const C64 a = u[1] / u[0], b = u[2] / u[3];
             const int ka = quarter_turn(a), kb = quarter_turn(b);
-            if (ka >= 0 && kb >= 0) {
-                // Off-diagonals in {1, i, -1, -i} (to 1e-14): adds only.
+            if (ka >= 0 && kb == ((2 - ka) & 3)) {
+                // Off-diagonals i^k and i^(2-k) = -conj(i^k) (to 1e-14): an
+                // add and an FMA by 2 per part, in place.
                 emit_u1(q, {C64(1, 0), a, b, C64(1, 0)}, FORM_UNIT_ROT, {});
-                out.back().aux = (uint32_t)(ka | (kb << 2));
+                out.back().aux = (uint32_t)ka;
             } else {
                 emit_u1(q, {C64(1, 0), a, b, C64(1, 0)}, FORM_UNIT_DIAG, {});
             }

@@ -45,7 +45,7 @@ constexpr uint32_t VF_GCTRL = 1u << 9;  // controls on non-tile qubits (gmask no
 //   FORM_FULL      [[c0, c1], [c2, c3]]
 //   FORM_UNIT_DIAG [[1, c1], [c2, 1]]   (4 fused multiply-adds per amplitude)
 //   FORM_UNIT_ANTI [[c0, 1], [1, c3]]
-//   FORM_UNIT_ROT  [[1, i^k1], [i^k2, 1]]   (adds only; k1 = aux & 3, k2 = aux >> 2)
+//   FORM_UNIT_ROT  [[1, i^k], [i^(2-k), 1]]  (an add and an FMA by 2 per part; k = aux & 3)
 enum VForm : uint32_t { FORM_FULL = 0, FORM_UNIT_DIAG = 1, FORM_UNIT_ANTI = 2, FORM_UNIT_ROT = 3 };
 
 constexpr uint32_t VT_THREAD = 0u << 16;  // target bit is bit x of the thread index

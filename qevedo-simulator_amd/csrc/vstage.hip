@@ -200,17 +200,44 @@ __device__ __forceinline__ void signs(T2 (&x)[1 << RB], const QVD_CONST VInstr &
     for (int j = 0; j < (1 << RB); j++) x[j] = xor_sign(x[j], t[j] ^ ((uint32_t)((neg >> j) & 1) << 31));
 }
 
-// a + i^K b: adds only.
+// The unit-form pair (a, b) -> (a + i^K b, b + i^(2-K) a) (the second
+// rotation is fixed by unitarity: M = [[1, m], [-conj(m), 1]]), computed in
+// place: the second output is rebuilt from the first and b
+// (b + i^(2-K) a = i^(2-K) o0 + 2 b, whose rotation only swaps and negates
+// parts), so both outputs can take the registers of their inputs and no
+// copy follows. One add and one FMA by 2 per part; the rounding differs from
+// the direct form by at most one unit of the inputs' magnitude.
+__device__ __forceinline__ float twice_plus(float b, float o) { return __builtin_fmaf(2.0f, b, o); }
+__device__ __forceinline__ float twice_minus(float b, float o) { return __builtin_fmaf(2.0f, b, -o); }
+__device__ __forceinline__ double twice_plus(double b, double o) { return __builtin_fma(2.0, b, o); }
+__device__ __forceinline__ double twice_minus(double b, double o) { return __builtin_fma(2.0, b, -o); }
 template <int K, typename T2>
-__device__ __forceinline__ T2 addrot(const T2 a, const T2 b) {
-    if (K == 0) return mk<T2>(a.x + b.x, a.y + b.y);
-    if (K == 1) return mk<T2>(a.x - b.y, a.y + b.x);
-    if (K == 2) return mk<T2>(a.x - b.x, a.y - b.y);
-    return mk<T2>(a.x + b.y, a.y - b.x);
+__device__ __forceinline__ void rotpair(T2 &a, T2 &b) {
+    if (K == 0) {  // (a + b, b - a)
+        a.x = a.x + b.x;
+        b.x = twice_minus(b.x, a.x);
+        a.y = a.y + b.y;
+        b.y = twice_minus(b.y, a.y);
+    } else if (K == 2) {  // (a - b, b + a)
+        a.x = a.x - b.x;
+        b.x = twice_plus(b.x, a.x);
+        a.y = a.y - b.y;
+        b.y = twice_plus(b.y, a.y);
+    } else if (K == 1) {  // (a + i b, b + i a)
+        a.x = a.x - b.y;
+        b.y = twice_plus(b.y, a.x);
+        a.y = a.y + b.x;
+        b.x = twice_minus(b.x, a.y);
+    } else {  // (a - i b, b - i a)
+        a.x = a.x + b.y;
+        b.y = twice_minus(b.y, a.x);
+        a.y = a.y - b.x;
+        b.x = twice_plus(b.x, a.y);
+    }
 }
 
 // 1-qubit gate on register bit B.
-template <typename T2, int RB, int B, int FORM, bool CTRL, int K1 = 0, int K2 = 0>
+template <typename T2, int RB, int B, int FORM, bool CTRL, int K1 = 0>
 __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t tpred_ok, uint32_t rmask,
                                     uint32_t rval) {
     T2 c0, c1, c2, c3;
@@ -226,12 +253,13 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uin
         if (j0 & (1 << B)) continue;
         const int j1 = j0 | (1 << B);
         if (CTRL && ((uint32_t)j0 & rmask) != rval) continue;
+        if (FORM == FORM_UNIT_ROT) {
+            rotpair<K1>(x[j0], x[j1]);
+            continue;
+        }
         const T2 a = x[j0], b = x[j1];
         T2 o0, o1;
-        if (FORM == FORM_UNIT_ROT) {
-            o0 = addrot<K1>(a, b);
-            o1 = addrot<K2>(b, a);
-        } else if (FORM == FORM_UNIT_DIAG) {
+        if (FORM == FORM_UNIT_DIAG) {
             o0 = cfma(c1, b, a);
             o1 = cfma(c2, a, b);
         } else if (FORM == FORM_UNIT_ANTI) {
@@ -250,36 +278,30 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uin
     }
 }
 
-template <typename T2, int RB, int FORM, bool CTRL, int K1 = 0, int K2 = 0>
+template <typename T2, int RB, int FORM, bool CTRL, int K1 = 0>
 __device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const QVD_CONST T2 *c, uint32_t tp,
                                          uint32_t rmask, uint32_t rval) {
     switch (slot) {
-        case 0: u1r<T2, RB, 0, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
-        case 1: u1r<T2, RB, 1, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
-        case 2: u1r<T2, RB, 2, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
-        case 3: u1r<T2, RB, 3, FORM, CTRL, K1, K2>(x, c, tp, rmask, rval); break;
+        case 0: u1r<T2, RB, 0, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
+        case 1: u1r<T2, RB, 1, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
+        case 2: u1r<T2, RB, 2, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
+        case 3: u1r<T2, RB, 3, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
         default:
-            if constexpr (RB > 4) u1r<T2, RB, (RB > 4 ? 4 : 0), FORM, CTRL, K1, K2>(x, c, tp, rmask, rval);
+            if constexpr (RB > 4) u1r<T2, RB, (RB > 4 ? 4 : 0), FORM, CTRL, K1>(x, c, tp, rmask, rval);
             break;
     }
 }
 
-// FORM_UNIT_ROT on a register bit: k = k1 | (k2 << 2).
+// FORM_UNIT_ROT on a register bit: the off-diagonal is i^k (k < 4).
 template <typename T2, int RB>
 __device__ __forceinline__ void u1r_rot(T2 (&x)[1 << RB], uint32_t slot, uint32_t k) {
     const QVD_CONST T2 *none = nullptr;
-#define QVD_ROT_BODY(K1, K2) u1r_slot<T2, RB, FORM_UNIT_ROT, false, K1, K2>(x, slot, none, 1, 0, 0); break;
-#define QVD_ROT(K1, K2) \
-    case (K1 | (K2 << 2)): QVD_ROT_BODY(K1, K2)
-    switch (k) {  // k < 16: the last combination takes the default arm
-        QVD_ROT(0, 0) QVD_ROT(1, 0) QVD_ROT(2, 0) QVD_ROT(3, 0)
-        QVD_ROT(0, 1) QVD_ROT(1, 1) QVD_ROT(2, 1) QVD_ROT(3, 1)
-        QVD_ROT(0, 2) QVD_ROT(1, 2) QVD_ROT(2, 2) QVD_ROT(3, 2)
-        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3)
-        default: QVD_ROT_BODY(3, 3)
+    switch (k & 3) {
+        case 0: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 0>(x, slot, none, 1, 0, 0); break;
+        case 1: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 1>(x, slot, none, 1, 0, 0); break;
+        case 2: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 2>(x, slot, none, 1, 0, 0); break;
+        default: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 3>(x, slot, none, 1, 0, 0); break;
     }
-#undef QVD_ROT
-#undef QVD_ROT_BODY
 }
 
 // 1-qubit gate on lane bit B: the partner amplitude comes from lane ^ (1 << B).
@@ -307,7 +329,7 @@ __device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const QVD_CONST T2 *c,
 // so that each lane holds a whole amplitude pair (of register 2i in the lanes
 // with bit B clear, of 2i + 1 in the others), the gate runs with uniform
 // coefficients as on a register bit, and a second swap puts the results back.
-template <typename T2, int RB, int FORM, bool CTRL, int B, int K1 = 0, int K2 = 0>
+template <typename T2, int RB, int FORM, bool CTRL, int B, int K1 = 0>
 __device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t lane, uint32_t tp,
                                          uint32_t rmask, uint32_t rval) {
     T2 c0, c1, c2, c3;
@@ -324,11 +346,15 @@ __device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const QVD_CONST T2 *c
         if (CTRL && (rmask & 1) == 0 && (((uint32_t)j & rmask) != rval)) continue;  // neither register selected
         T2 a = x[j], b = x[j + 1];
         row_swap2<B>(a, b);
-        T2 o0, o1;
         if (FORM == FORM_UNIT_ROT) {
-            o0 = addrot<K1>(a, b);
-            o1 = addrot<K2>(b, a);
-        } else if (FORM == FORM_UNIT_DIAG) {
+            rotpair<K1>(a, b);
+            row_swap2<B>(a, b);
+            x[j] = a;
+            x[j + 1] = b;
+            continue;
+        }
+        T2 o0, o1;
+        if (FORM == FORM_UNIT_DIAG) {
             o0 = cfma(c1, b, a);
             o1 = cfma(c2, a, b);
         } else if (FORM == FORM_UNIT_ANTI) {
@@ -362,23 +388,17 @@ __device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONS
     }
 }
 
-// FORM_UNIT_ROT on lane bit 4 or 5 (k = k1 | (k2 << 2)); other lane bits take
+// FORM_UNIT_ROT on lane bit 4 or 5 (off-diagonal i^k); other lane bits take
 // the multiply-add form, since the two lanes of a pair need different rotations.
 template <typename T2, int RB, int B>
 __device__ __forceinline__ void u1l_rot_pair(T2 (&x)[1 << RB], uint32_t lane, uint32_t k) {
     const QVD_CONST T2 *none = nullptr;
-#define QVD_ROT_BODY(K1, K2) u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K1, K2>(x, none, lane, 1, 0, 0); break;
-#define QVD_ROT(K1, K2) \
-    case (K1 | (K2 << 2)): QVD_ROT_BODY(K1, K2)
-    switch (k) {  // k < 16: the last combination takes the default arm
-        QVD_ROT(0, 0) QVD_ROT(1, 0) QVD_ROT(2, 0) QVD_ROT(3, 0)
-        QVD_ROT(0, 1) QVD_ROT(1, 1) QVD_ROT(2, 1) QVD_ROT(3, 1)
-        QVD_ROT(0, 2) QVD_ROT(1, 2) QVD_ROT(2, 2) QVD_ROT(3, 2)
-        QVD_ROT(0, 3) QVD_ROT(1, 3) QVD_ROT(2, 3)
-        default: QVD_ROT_BODY(3, 3)
+    switch (k & 3) {
+        case 0: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 0>(x, none, lane, 1, 0, 0); break;
+        case 1: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 1>(x, none, lane, 1, 0, 0); break;
+        case 2: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 2>(x, none, lane, 1, 0, 0); break;
+        default: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 3>(x, none, lane, 1, 0, 0); break;
     }
-#undef QVD_ROT
-#undef QVD_ROT_BODY
 }
 
 // A run of pair phases: amplitude (thread, register j) gains w for every
