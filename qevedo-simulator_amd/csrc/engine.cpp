@@ -154,6 +154,18 @@ template <typename T2> T2 to_t2(C64 v) {
     return r;
 }
 
+// Copies each 1-qubit gate's four coefficients (already in the state's
+// precision) into the instruction, where the kernel's look-ahead fetch reads
+// them with the rest of the instruction.
+template <typename T2> void inline_coefficients(std::vector<VInstr> &ins, const std::vector<T2> &coefs) {
+    static_assert(4 * sizeof(T2) <= sizeof(VInstr::inl), "inline coefficient block too small");
+    for (VInstr &v : ins) {
+        const uint32_t op = v.op & VOP_MASK;
+        if ((op != V_U1R && op != V_U1L) || (size_t)v.coef + 4 > coefs.size()) continue;
+        std::memcpy(v.inl, &coefs[v.coef], 4 * sizeof(T2));
+    }
+}
+
 // One op as its own HBM pass: the dense kernel over the controlled subspace,
 // or the diagonal kernel.
 int g_num_cus = 0;
@@ -372,6 +384,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
     }
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
+    inline_coefficients(vins, vco_t);
     DBuf dvins(std::max<size_t>(vins.size(), 1) * sizeof(VInstr));
     DBuf dvco(std::max<size_t>(vco_t.size(), 1) * sizeof(T2));
     DBuf dvlds(std::max<size_t>(vlds.size(), 1) * sizeof(StageOp));
@@ -844,6 +857,7 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     // Upload.
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
+    inline_coefficients(vins, vco_t);
     auto upload = [&](DBuf &buf, const void *src, size_t bytes) {
         buf.alloc(std::max<size_t>(bytes, 16));
         if (bytes) QVD_HIP(hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice));

@@ -635,8 +635,38 @@ VProgram build_vprogram(const Stage &stage, uint32_t n, const VGeom &g, PendingD
     for (VInstr &v : p.instr) {
         if (v.tmask != 0 || v.rmask != 0) v.op |= VF_CTRL;
         if (v.gmask != 0) v.op |= VF_GCTRL;
+        v.op |= vhandler(v) << VH_SHIFT;
     }
     return p;
+}
+
+uint32_t vhandler(const VInstr &v) {
+    const uint32_t op = v.op & VOP_MASK;
+    const bool ctrl = (v.op & VF_CTRL) != 0;
+    switch (op) {
+        case V_U1R:
+            require(v.slot < 4, "register bit out of range");
+            if (ctrl) return H_U1R_CTRL + v.slot;
+            if (v.form == FORM_UNIT_ROT) return H_U1R_ROT + 4 * v.slot + (v.aux & 3);
+            if (v.form == FORM_UNIT_DIAG) return H_U1R_DIAG + v.slot;
+            if (v.form == FORM_UNIT_ANTI) return H_U1R_ANTI + v.slot;
+            return H_U1R_FULL + v.slot;
+        case V_U1L:
+            require(v.slot < 6, "lane bit out of range");
+            if (ctrl) return H_U1L_CTRL + v.slot;
+            if (v.form == FORM_UNIT_ROT && v.slot >= 4) return H_U1L_ROT + 4 * (v.slot - 4) + (v.aux & 3);
+            if (v.form == FORM_UNIT_DIAG || v.form == FORM_UNIT_ROT) return H_U1L_DIAG + v.slot;
+            if (v.form == FORM_UNIT_ANTI) return H_U1L_ANTI + v.slot;
+            return H_U1L_FULL + v.slot;
+        case V_DIAG: return H_DIAG + 2 * (v.threaded ? 1 : 0) + (v.form == 1 ? 1 : 0);
+        case V_SIGNS: return H_SIGNS;
+        case V_PHASES: return H_PHASES;
+        case V_SCALE: return v.form == 1 ? H_SCALE1 : H_SCALE;
+        case V_XCHG: return H_XCHG;
+        case V_LDSOP: return H_LDSOP;
+        default: require(false, "unknown register-tile instruction");
+    }
+    return H_COUNT;
 }
 
 }  // namespace qvd

@@ -15,6 +15,8 @@
 // workgroup runs the same program on each of its tiles.
 #pragma once
 
+#include <cstddef>
+
 #include <cstdint>
 #include <vector>
 
@@ -35,11 +37,37 @@ enum VOp : uint32_t {
     V_PHASES = 7, // a run of commuting pair phases: factor = product of w over pairs with both bits 1
 };
 
-// VInstr::op holds the VOp in its low byte and these flags above it, so the
-// kernel reads one word to dispatch and loads other fields only where used.
+// VInstr::op holds the VOp in its low byte, these flags above it and, in
+// bits 16-23, the kernel code path (VHandler) that runs the instruction, so
+// the kernel reads one word and branches once to dispatch, and loads other
+// fields only where used.
 constexpr uint32_t VOP_MASK = 0xffu;
 constexpr uint32_t VF_CTRL = 1u << 8;   // controls on thread or register bits (tmask / rmask nonzero)
 constexpr uint32_t VF_GCTRL = 1u << 9;  // controls on non-tile qubits (gmask nonzero)
+constexpr uint32_t VH_SHIFT = 16;
+
+// Kernel code paths. Each fixes at compile time what the VOp, form, slot and
+// rotation leave open, so an instruction costs one switch in the kernel.
+enum VHandler : uint32_t {
+    H_U1R_ROT = 0,    // + 4 * register bit + k          (FORM_UNIT_ROT)
+    H_U1R_DIAG = 16,  // + register bit
+    H_U1R_ANTI = 20,  // + register bit
+    H_U1R_FULL = 24,  // + register bit
+    H_U1R_CTRL = 28,  // + register bit                  (full form, controls)
+    H_U1L_ROT = 32,   // + 4 * (lane bit - 4) + k        (lane bits 4 and 5)
+    H_U1L_DIAG = 40,  // + lane bit                      (also FORM_UNIT_ROT on lane bits 0-3)
+    H_U1L_ANTI = 46,  // + lane bit
+    H_U1L_FULL = 52,  // + lane bit
+    H_U1L_CTRL = 58,  // + lane bit                      (full form, controls)
+    H_DIAG = 64,      // + 2 * threaded + (form == 1)
+    H_SIGNS = 68,
+    H_PHASES = 69,
+    H_SCALE1 = 70,    // SCALE form 1: one uniform factor
+    H_SCALE = 71,
+    H_XCHG = 72,
+    H_LDSOP = 73,
+    H_COUNT = 74,
+};
 
 // Forms of a 1-qubit gate's coefficients c[0..3]:
 //   FORM_FULL      [[c0, c1], [c2, c3]]
@@ -69,6 +97,10 @@ struct VInstr {
     uint64_t neg;          // DIAG, form 1: entries equal to -1 (the others are +1)
     uint32_t coef;         // first coefficient (complex) of the op
     uint32_t aux;          // LDSOP: index of its StageOp; SCALE: first non-tile qubit in the int array
+    // U1R / U1L: the four matrix coefficients in the state's precision (4 x
+    // float2 or 4 x double2), written when the program is uploaded, so the
+    // kernel reads them from the instruction instead of following coef.
+    uint32_t inl[16];
     uint32_t tgt[MAX_GATE_QUBITS];            // DIAG targets (VT_* | bit)
     uint32_t threaded;                        // DIAG: some target or control is a thread-index bit
     uint8_t rtab[VMAX_REGS];                  // DIAG: register-index part of the entry index, per register
@@ -77,6 +109,13 @@ struct VInstr {
     uint8_t rmap[VMAX_R_BITS];                // XCHG: tile bit of each register bit
     uint8_t pad[2];
 };
+
+// The handler of an instruction (VF_* flags and fields already set); the
+// register-bit count must be at most 4.
+uint32_t vhandler(const VInstr &v);
+
+static_assert(offsetof(VInstr, coef) == 56 && offsetof(VInstr, aux) == 60 && offsetof(VInstr, inl) == 64,
+              "the kernel reads an instruction's first words by position");
 
 // One register-tile stage as launched. A launch runs one or more of them
 // (distinct states of a batch), each on its own range of workgroups.

@@ -20,6 +20,7 @@
 #include "vstage.h"
 
 #include <atomic>
+#include <type_traits>
 
 // Constant address space: loads through these pointers are invariant, so
 // uniform ones go through the scalar cache (s_load) and never wait behind
@@ -33,6 +34,16 @@
 namespace qvd {
 
 namespace {
+
+// Byte i of a byte array in constant memory, read as part of its aligned
+// 32-bit word: the scalar unit loads whole words only, and a byte load would
+// take the vector memory path, whose wait counter the tile's own loads and
+// stores share.
+__device__ __forceinline__ uint32_t cbyte(const QVD_CONST uint8_t *p, uint32_t i) {
+    const uintptr_t a = (uintptr_t)(p + i);
+    const QVD_CONST uint32_t *w = (const QVD_CONST uint32_t *)(a & ~uintptr_t(3));
+    return (*w >> (8 * (uint32_t)(a & 3))) & 0xffu;
+}
 
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -238,7 +249,7 @@ __device__ __forceinline__ void rotpair(T2 &a, T2 &b) {
 
 // 1-qubit gate on register bit B.
 template <typename T2, int RB, int B, int FORM, bool CTRL, int K1 = 0>
-__device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t tpred_ok, uint32_t rmask,
+__device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const T2 *c, uint32_t tpred_ok, uint32_t rmask,
                                     uint32_t rval) {
     T2 c0, c1, c2, c3;
     if (FORM != FORM_UNIT_ROT) {
@@ -278,39 +289,13 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uin
     }
 }
 
-template <typename T2, int RB, int FORM, bool CTRL, int K1 = 0>
-__device__ __forceinline__ void u1r_slot(T2 (&x)[1 << RB], uint32_t slot, const QVD_CONST T2 *c, uint32_t tp,
-                                         uint32_t rmask, uint32_t rval) {
-    switch (slot) {
-        case 0: u1r<T2, RB, 0, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
-        case 1: u1r<T2, RB, 1, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
-        case 2: u1r<T2, RB, 2, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
-        case 3: u1r<T2, RB, 3, FORM, CTRL, K1>(x, c, tp, rmask, rval); break;
-        default:
-            if constexpr (RB > 4) u1r<T2, RB, (RB > 4 ? 4 : 0), FORM, CTRL, K1>(x, c, tp, rmask, rval);
-            break;
-    }
-}
-
-// FORM_UNIT_ROT on a register bit: the off-diagonal is i^k (k < 4).
-template <typename T2, int RB>
-__device__ __forceinline__ void u1r_rot(T2 (&x)[1 << RB], uint32_t slot, uint32_t k) {
-    const QVD_CONST T2 *none = nullptr;
-    switch (k & 3) {
-        case 0: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 0>(x, slot, none, 1, 0, 0); break;
-        case 1: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 1>(x, slot, none, 1, 0, 0); break;
-        case 2: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 2>(x, slot, none, 1, 0, 0); break;
-        default: u1r_slot<T2, RB, FORM_UNIT_ROT, false, 3>(x, slot, none, 1, 0, 0); break;
-    }
-}
-
 // 1-qubit gate on lane bit B: the partner amplitude comes from lane ^ (1 << B).
 template <typename T2, int RB, int FORM, bool CTRL, int B>
-__device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t lane, uint32_t tp,
+__device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const T2 *c, uint32_t lane, uint32_t tp,
                                         uint32_t rmask, uint32_t rval) {
     const bool hi = (lane >> B) & 1;
-    const T2 cs = hi ? c[3] : c[0];  // coefficient of the lane's own amplitude
-    const T2 cp = hi ? c[2] : c[1];  // coefficient of the partner's
+    const T2 cs = sel(hi, c[3], c[0]);  // coefficient of the lane's own amplitude
+    const T2 cp = sel(hi, c[2], c[1]);  // coefficient of the partner's
 #pragma unroll
     for (int j = 0; j < (1 << RB); j++) {
         if (CTRL && ((uint32_t)j & rmask) != rval) continue;
@@ -330,7 +315,7 @@ __device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const QVD_CONST T2 *c,
 // with bit B clear, of 2i + 1 in the others), the gate runs with uniform
 // coefficients as on a register bit, and a second swap puts the results back.
 template <typename T2, int RB, int FORM, bool CTRL, int B, int K1 = 0>
-__device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const QVD_CONST T2 *c, uint32_t lane, uint32_t tp,
+__device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const T2 *c, uint32_t lane, uint32_t tp,
                                          uint32_t rmask, uint32_t rval) {
     T2 c0, c1, c2, c3;
     if (FORM != FORM_UNIT_ROT) {
@@ -372,32 +357,6 @@ __device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const QVD_CONST T2 *c
         row_swap2<B>(o0, o1);
         x[j] = o0;
         x[j + 1] = o1;
-    }
-}
-
-template <typename T2, int RB, int FORM, bool CTRL>
-__device__ __forceinline__ void u1l(T2 (&x)[1 << RB], uint32_t b, const QVD_CONST T2 *c, uint32_t lane,
-                                    uint32_t tp, uint32_t rmask, uint32_t rval) {
-    switch (b) {
-        case 0: u1l_bit<T2, RB, FORM, CTRL, 0>(x, c, lane, tp, rmask, rval); break;
-        case 1: u1l_bit<T2, RB, FORM, CTRL, 1>(x, c, lane, tp, rmask, rval); break;
-        case 2: u1l_bit<T2, RB, FORM, CTRL, 2>(x, c, lane, tp, rmask, rval); break;
-        case 3: u1l_bit<T2, RB, FORM, CTRL, 3>(x, c, lane, tp, rmask, rval); break;
-        case 4: u1l_pair<T2, RB, FORM, CTRL, 4>(x, c, lane, tp, rmask, rval); break;
-        default: u1l_pair<T2, RB, FORM, CTRL, 5>(x, c, lane, tp, rmask, rval); break;
-    }
-}
-
-// FORM_UNIT_ROT on lane bit 4 or 5 (off-diagonal i^k); other lane bits take
-// the multiply-add form, since the two lanes of a pair need different rotations.
-template <typename T2, int RB, int B>
-__device__ __forceinline__ void u1l_rot_pair(T2 (&x)[1 << RB], uint32_t lane, uint32_t k) {
-    const QVD_CONST T2 *none = nullptr;
-    switch (k & 3) {
-        case 0: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 0>(x, none, lane, 1, 0, 0); break;
-        case 1: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 1>(x, none, lane, 1, 0, 0); break;
-        case 2: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 2>(x, none, lane, 1, 0, 0); break;
-        default: u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, 3>(x, none, lane, 1, 0, 0); break;
     }
 }
 
@@ -466,7 +425,7 @@ __device__ __forceinline__ void diag(T2 (&x)[1 << RB], const QVD_CONST VInstr &v
 #pragma unroll
     for (int j = 0; j < (1 << RB); j++) {
         if (((uint32_t)j & v.rmask) != v.rval) continue;
-        const uint32_t ij = ig | v.rtab[j];
+        const uint32_t ij = ig | cbyte(v.rtab, j);
         const T2 a = x[j];
         if (SIGN) {
             if (THREADED) {
@@ -521,6 +480,15 @@ __device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__res
     }
 }
 
+// A 1-qubit gate's four coefficients, read from the instruction (inline
+// copy in the state's precision).
+template <typename T2>
+__device__ __forceinline__ void inline_coefs(const QVD_CONST VInstr &v, T2 (&c)[4]) {
+    const QVD_CONST T2 *p = (const QVD_CONST T2 *)v.inl;
+#pragma unroll
+    for (int j = 0; j < 4; j++) c[j] = p[j];
+}
+
 }  // namespace
 
 template <typename T2, int WB, int RB, int FB>
@@ -568,20 +536,39 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     uint64_t t_off0 = lane & ((1u << FB) - 1), t_offF = t_off0;
 #pragma unroll
     for (int s_ = 0; s_ < (int)FL; s_++) {
-        t_off0 |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[d.lmap0[s_]];
-        t_offF |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[d.lmapF[s_]];
+        t_off0 |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[cbyte(d.lmap0, s_)];
+        t_offF |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[cbyte(d.lmapF, s_)];
     }
 #pragma unroll
     for (int s_ = 0; s_ < WB; s_++) {
-        t_off0 |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[d.wmap0[s_]];
-        t_offF |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[d.wmapF[s_]];
+        t_off0 |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[cbyte(d.wmap0, s_)];
+        t_offF |= (uint64_t)((tid >> (6 + s_)) & 1) << d.local_pos[cbyte(d.wmapF, s_)];
     }
     uint64_t p0[RB], pF[RB];
 #pragma unroll
     for (int b = 0; b < RB; b++) {
-        p0[b] = uint64_t(1) << d.local_pos[d.rmap0[b]];
-        pF[b] = uint64_t(1) << d.local_pos[d.rmapF[b]];
+        p0[b] = uint64_t(1) << d.local_pos[cbyte(d.rmap0, b)];
+        pF[b] = uint64_t(1) << d.local_pos[cbyte(d.rmapF, b)];
     }
+
+    // Role map at the start of every tile: the thread's part of the tile
+    // index (fixed lane bits, flexible lane bits, wave bits) and the
+    // register bits' tile bits (uniform).
+    auto thread_index = [&](const QVD_CONST uint8_t *lm, const QVD_CONST uint8_t *wmp) {
+        uint32_t ti = lane & ((1u << FB) - 1);
+#pragma unroll
+        for (int s_ = 0; s_ < (int)FL; s_++) ti |= ((lane >> (FB + s_)) & 1) << cbyte(lm, s_);
+#pragma unroll
+        for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << cbyte(wmp, s_);
+        return ti;
+    };
+    const uint32_t ti0 = thread_index(d.lmap0, d.wmap0);
+    uint32_t rm0[RB];
+#pragma unroll
+    for (int b = 0; b < RB; b++) rm0[b] = cbyte(d.rmap0, b);
+
+    const QVD_CONST VInstr *const ip = cprog + d.instr_off;
+    const QVD_CONST VInstr *const ip_end = ip + d.num_instr;
 
     // With 4 waves per SIMD (16-wave workgroups) the other waves cover a
     // tile's load latency; with 2 the next tile is prefetched into a second
@@ -605,145 +592,140 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         }
     };
 
+    // The first tile is loaded, and waited for, before the loop. With a
+    // prefetch set the loop then copies the next tile into x at the end of
+    // each iteration, after its stores; the wait for those loads leaves the
+    // stores in flight, and no register entering the loop has a load pending
+    // (the memory counter is shared by loads and stores, and a pending load
+    // at the loop head would make every tile wait for its predecessor's
+    // stores).
     uint64_t t = bl;
-    if constexpr (PREFETCH)
-        if (t < d.num_tiles) load(y, tile_base(t));
+    if (t < d.num_tiles) load(x, tile_base(t));
+    if constexpr (PREFETCH) __builtin_amdgcn_s_waitcnt(0);
     for (; t < d.num_tiles; t += nb) {
         const uint64_t base = tile_base(t);
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            if constexpr (PREFETCH) x[j] = y[j];
-        }
-        if constexpr (PREFETCH) {
-            const uint64_t tn = t + nb;
+        const uint64_t tn = t + nb;
+        if constexpr (PREFETCH)
             if (tn < d.num_tiles) load(y, tile_base(tn));
-        } else {
-            load(x, base);
-        }
 
-        // Current role map: the thread's part of the tile index (fixed lane
-        // bits, flexible lane bits, wave bits) and the register bits' tile
-        // bits (uniform).
-        auto thread_index = [&](const QVD_CONST uint8_t *lm, const QVD_CONST uint8_t *wmp) {
-            uint32_t ti = lane & ((1u << FB) - 1);
-#pragma unroll
-            for (int s_ = 0; s_ < (int)FL; s_++) ti |= ((lane >> (FB + s_)) & 1) << lm[s_];
-#pragma unroll
-            for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << wmp[s_];
-            return ti;
-        };
-        uint32_t ti = thread_index(d.lmap0, d.wmap0);
+        // Current role map (an exchange changes it; each tile starts from
+        // the stage's initial one).
+        uint32_t ti = ti0;
         uint32_t rm[RB];
 #pragma unroll
-        for (int b = 0; b < RB; b++) rm[b] = d.rmap0[b];
+        for (int b = 0; b < RB; b++) rm[b] = rm0[b];
 
-        for (uint32_t ii = 0; ii < d.num_instr; ii++) {
-            const QVD_CONST VInstr &v = cprog[d.instr_off + ii];
+        // The program, one instruction at a time: the host has resolved each
+        // instruction to a handler (VHandler), so dispatch is one switch and
+        // every arm is straight-line code with its register bits, forms and
+        // rotations fixed at compile time.
+        auto handle = [&](auto hc, const QVD_CONST VInstr &v) {
+            constexpr uint32_t H = decltype(hc)::value;
+            if constexpr (H < H_U1R_DIAG) {
+                constexpr int B = (H - H_U1R_ROT) / 4, K = (H - H_U1R_ROT) % 4;
+                u1r<T2, RB, B, FORM_UNIT_ROT, false, K>(x, nullptr, 1, 0, 0);
+            } else if constexpr (H < H_U1R_CTRL) {
+                constexpr int F = H < H_U1R_ANTI ? FORM_UNIT_DIAG : H < H_U1R_FULL ? FORM_UNIT_ANTI : FORM_FULL;
+                constexpr int B = (H - H_U1R_DIAG) % 4;
+                T2 c[4];
+                inline_coefs(v, c);
+                u1r<T2, RB, B, F, false>(x, c, 1, 0, 0);
+            } else if constexpr (H < H_U1L_ROT) {
+                constexpr int B = H - H_U1R_CTRL;
+                T2 c[4];
+                inline_coefs(v, c);
+                u1r<T2, RB, B, FORM_FULL, true>(x, c, (tid & v.tmask) == v.tval, v.rmask, v.rval);
+            } else if constexpr (H < H_U1L_DIAG) {
+                constexpr int B = 4 + (H - H_U1L_ROT) / 4, K = (H - H_U1L_ROT) % 4;
+                u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K>(x, nullptr, lane, 1, 0, 0);
+            } else if constexpr (H < H_DIAG) {
+                constexpr bool CTRL = H >= H_U1L_CTRL;
+                constexpr int F = CTRL ? FORM_FULL
+                                  : H < H_U1L_ANTI ? FORM_UNIT_DIAG : H < H_U1L_FULL ? FORM_UNIT_ANTI : FORM_FULL;
+                constexpr int B = (H - H_U1L_DIAG) % 6;
+                T2 c[4];
+                inline_coefs(v, c);
+                uint32_t tp = 1, rmask = 0, rval = 0;
+                if constexpr (CTRL) {
+                    tp = (tid & v.tmask) == v.tval;
+                    rmask = v.rmask;
+                    rval = v.rval;
+                }
+                if constexpr (B < 4) u1l_bit<T2, RB, F, CTRL, B>(x, c, lane, tp, rmask, rval);
+                else u1l_pair<T2, RB, F, CTRL, B>(x, c, lane, tp, rmask, rval);
+            } else if constexpr (H < H_SIGNS) {
+                constexpr bool THREADED = ((H - H_DIAG) >> 1) != 0, SIGN = ((H - H_DIAG) & 1) != 0;
+                diag<T2, RB, SIGN, THREADED>(x, v, cbase, tid, base);
+            } else if constexpr (H == H_SIGNS) {
+                signs<T2, RB>(x, v, cints, tid, base);
+            } else if constexpr (H == H_PHASES) {
+                phases<T2, RB>(x, v, ccoef, cints, tid, base);
+            } else if constexpr (H == H_SCALE1) {
+                const T2 f = ccoef[v.coef];
+#pragma unroll
+                for (int j = 0; j < R; j++) x[j] = cm(f, x[j]);
+            } else if constexpr (H == H_SCALE) {
+                T2 ft = cbase[v.coef + tid];
+                const T2 *g = cbase + v.coef + W + R;
+                for (uint32_t i = 0; i < v.k; i++) {
+                    const uint32_t q = cints[v.aux + i];
+                    ft = cm(g[2 * i + ((base >> q) & 1)], ft);
+                }
+#pragma unroll
+                for (int j = 0; j < R; j++) x[j] = cm(cm(ccoef[v.coef + W + j], ft), x[j]);
+            } else if constexpr (H == H_XCHG || H == H_LDSOP) {
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    uint32_t o = 0;
+#pragma unroll
+                    for (int b = 0; b < RB; b++)
+                        if ((j >> b) & 1) o |= 1u << rm[b];
+                    tile[ti | o] = x[j];
+                }
+                __syncthreads();
+                if constexpr (H == H_LDSOP) {
+                    const QVD_CONST StageOp &op = cops[v.aux];
+                    const T2 *m = cbase + op.coef_off;
+                    switch (op.k) {
+                        case 2: lds_dense<T2, 2, W>(tile, m, op, L); break;
+                        case 3: lds_dense<T2, 3, W>(tile, m, op, L); break;
+                        case 4: lds_dense<T2, 4, W>(tile, m, op, L); break;
+                        default: lds_dense<T2, 1, W>(tile, m, op, L); break;
+                    }
+                    __syncthreads();
+                } else {
+#pragma unroll
+                    for (int b = 0; b < RB; b++) rm[b] = cbyte(v.rmap, b);
+                    ti = thread_index(v.lmap, v.wmap);
+                }
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    uint32_t o = 0;
+#pragma unroll
+                    for (int b = 0; b < RB; b++)
+                        if ((j >> b) & 1) o |= 1u << rm[b];
+                    x[j] = tile[ti | o];
+                }
+            }
+        };
+        static_assert(H_COUNT == 74, "the switch below lists every handler");
+#define QVD_H(n) \
+    case (n): handle(std::integral_constant<uint32_t, (n)>{}, v); break;
+#define QVD_H4(n) QVD_H(n) QVD_H((n) + 1) QVD_H((n) + 2) QVD_H((n) + 3)
+        for (const QVD_CONST VInstr *pv = ip; pv != ip_end; ++pv) {
+            const QVD_CONST VInstr &v = *pv;
             const uint32_t w = v.op;
             if ((w & VF_GCTRL) && (base & v.gmask) != v.gval) continue;
-            const bool ctrl = (w & VF_CTRL) != 0;
-            switch (w & VOP_MASK) {
-                case V_U1R: {
-                    const QVD_CONST T2 *c = ccoef + v.coef;
-                    if (!ctrl) {
-                        if (v.form == FORM_UNIT_ROT) u1r_rot<T2, RB>(x, v.slot, v.aux);
-                        else if (v.form == FORM_UNIT_DIAG) u1r_slot<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, 1, 0, 0);
-                        else if (v.form == FORM_UNIT_ANTI) u1r_slot<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, 1, 0, 0);
-                        else u1r_slot<T2, RB, FORM_FULL, false>(x, v.slot, c, 1, 0, 0);
-                    } else {
-                        const uint32_t tp = (tid & v.tmask) == v.tval;
-                        u1r_slot<T2, RB, FORM_FULL, true>(x, v.slot, c, tp, v.rmask, v.rval);
-                    }
-                    break;
-                }
-                case V_U1L: {
-                    const QVD_CONST T2 *c = ccoef + v.coef;
-                    if (!ctrl) {
-                        if (v.form == FORM_UNIT_ROT && v.slot >= 4) {
-                            if (v.slot == 4) u1l_rot_pair<T2, RB, 4>(x, lane, v.aux);
-                            else u1l_rot_pair<T2, RB, 5>(x, lane, v.aux);
-                        } else if (v.form == FORM_UNIT_DIAG || v.form == FORM_UNIT_ROT)
-                            u1l<T2, RB, FORM_UNIT_DIAG, false>(x, v.slot, c, lane, 1, 0, 0);
-                        else if (v.form == FORM_UNIT_ANTI) u1l<T2, RB, FORM_UNIT_ANTI, false>(x, v.slot, c, lane, 1, 0, 0);
-                        else u1l<T2, RB, FORM_FULL, false>(x, v.slot, c, lane, 1, 0, 0);
-                    } else {
-                        const uint32_t tp = (tid & v.tmask) == v.tval;
-                        u1l<T2, RB, FORM_FULL, true>(x, v.slot, c, lane, tp, v.rmask, v.rval);
-                    }
-                    break;
-                }
-                case V_DIAG:
-                    if (v.threaded) {
-                        if (v.form == 1) diag<T2, RB, true, true>(x, v, cbase, tid, base);
-                        else diag<T2, RB, false, true>(x, v, cbase, tid, base);
-                    } else {
-                        if (v.form == 1) diag<T2, RB, true, false>(x, v, cbase, tid, base);
-                        else diag<T2, RB, false, false>(x, v, cbase, tid, base);
-                    }
-                    break;
-                case V_SIGNS:
-                    signs<T2, RB>(x, v, cints, tid, base);
-                    break;
-                case V_PHASES:
-                    phases<T2, RB>(x, v, ccoef, cints, tid, base);
-                    break;
-                case V_SCALE: {
-                    if (v.form == 1) {
-                        const T2 f = ccoef[v.coef];
-#pragma unroll
-                        for (int j = 0; j < R; j++) x[j] = cm(f, x[j]);
-                    } else {
-                        T2 ft = cbase[v.coef + tid];
-                        const T2 *g = cbase + v.coef + W + R;
-                        for (uint32_t i = 0; i < v.k; i++) {
-                            const uint32_t q = cints[v.aux + i];
-                            ft = cm(g[2 * i + ((base >> q) & 1)], ft);
-                        }
-#pragma unroll
-                        for (int j = 0; j < R; j++) x[j] = cm(cm(ccoef[v.coef + W + j], ft), x[j]);
-                    }
-                    break;
-                }
-                case V_XCHG:
-                case V_LDSOP: {
-                    __syncthreads();
-#pragma unroll
-                    for (int j = 0; j < R; j++) {
-                        uint32_t o = 0;
-#pragma unroll
-                        for (int b = 0; b < RB; b++)
-                            if ((j >> b) & 1) o |= 1u << rm[b];
-                        tile[ti | o] = x[j];
-                    }
-                    __syncthreads();
-                    if ((w & VOP_MASK) == V_LDSOP) {
-                        const QVD_CONST StageOp &op = cops[v.aux];
-                        const T2 *m = cbase + op.coef_off;
-                        switch (op.k) {
-                            case 2: lds_dense<T2, 2, W>(tile, m, op, L); break;
-                            case 3: lds_dense<T2, 3, W>(tile, m, op, L); break;
-                            case 4: lds_dense<T2, 4, W>(tile, m, op, L); break;
-                            default: lds_dense<T2, 1, W>(tile, m, op, L); break;
-                        }
-                        __syncthreads();
-                    } else {
-#pragma unroll
-                        for (int b = 0; b < RB; b++) rm[b] = v.rmap[b];
-                        ti = thread_index(v.lmap, v.wmap);
-                    }
-#pragma unroll
-                    for (int j = 0; j < R; j++) {
-                        uint32_t o = 0;
-#pragma unroll
-                        for (int b = 0; b < RB; b++)
-                            if ((j >> b) & 1) o |= 1u << rm[b];
-                        x[j] = tile[ti | o];
-                    }
-                    break;
-                }
+            switch ((w >> VH_SHIFT) & 0xffu) {
+                QVD_H4(0) QVD_H4(4) QVD_H4(8) QVD_H4(12) QVD_H4(16) QVD_H4(20) QVD_H4(24) QVD_H4(28)
+                QVD_H4(32) QVD_H4(36) QVD_H4(40) QVD_H4(44) QVD_H4(48) QVD_H4(52) QVD_H4(56) QVD_H4(60)
+                QVD_H4(64) QVD_H4(68) QVD_H(72) QVD_H(73)
                 default: break;
             }
         }
+#undef QVD_H4
+#undef QVD_H
 
         T2 *dst = s + base + t_offF;
 #pragma unroll
@@ -756,6 +738,12 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         }
         // The next tile's first exchange writes LDS only after a barrier, so
         // no barrier is needed here.
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int j = 0; j < R; j++) x[j] = y[j];
+        } else if (tn < d.num_tiles) {
+            load(x, tile_base(tn));
+        }
     }
 }
 
