@@ -129,6 +129,50 @@ def run_precision_form(n: int, depth: int, seed: int) -> dict:
             "c128_seconds": c128_s, "max_abs": err["max_abs"], "rel_l2": err["rel_l2"]}
 
 
+def run_learned_form(n: int, steps: int, seed: int, model_path: str, reps: int = 2) -> dict:
+    """Learned form: a kicked-Ising Trotter circuit (RX(theta) on every qubit,
+    theta uniform in [0.1, 1.5], then RZZ(-pi/2) on a heavy-hex-like graph),
+    run with the heuristic planner and with the learned one; both final
+    states are kept and compared on the device."""
+    import numpy as np
+
+    from qevedo.simulator import StateVector, learned, library, lower_gates
+    from qevedo.simulator import _native
+
+    theta = float(np.random.Generator(np.random.PCG64(seed)).uniform(0.1, 1.5))
+    gates = library.kicked_ising(n, library.heavy_hex_edges(n), steps, theta)
+    ops = _native.OpArray(lower_gates(gates))
+    times = {}
+    states = {}
+    net = learned.install(learned.load(model_path))
+    try:
+        for name, planner in (("heuristic", _native.PLANNER_HEURISTIC), ("learned", _native.PLANNER_LEARNED)):
+            sv = StateVector(n, "single")
+            st = sv.apply_ops(ops, 4, 0, planner=planner)  # warm-up (and plan choice)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                sv.set_zero()
+                st = sv.apply_ops(ops, 4, 0, planner=planner)
+            times[name] = (time.perf_counter() - t0) / reps
+            times[name + "_stages"] = st["stages"]
+            times[name + "_variant"] = st["plan_variant"]
+            states[name] = sv
+        diff = states["learned"].compare(states["heuristic"])
+    finally:
+        for sv in states.values():
+            sv.close()
+        _native.clear_planner_model()
+        net.close()
+    log(f"learned form {n} q kicked Ising ({len(gates)} gates, theta {theta:.3f}): heuristic {times['heuristic']:.3f} s "
+        f"({times['heuristic_stages']} stages), learned {times['learned']:.3f} s (variant {times['learned_variant']}, "
+        f"{times['learned_stages']} stages): {times['heuristic'] / times['learned']:.3f}x; "
+        f"states differ by max-abs {diff['max_abs']:.2e}")
+    return {"qubits": n, "steps": steps, "theta": theta, "gates": len(gates), "heuristic_seconds": times["heuristic"],
+            "learned_seconds": times["learned"], "speedup": times["heuristic"] / times["learned"],
+            "learned_variant": times["learned_variant"], "heuristic_stages": times["heuristic_stages"],
+            "learned_stages": times["learned_stages"], "max_abs_diff": diff["max_abs"], "rel_l2_diff": diff["rel_l2"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +192,8 @@ def main() -> int:
     ap.add_argument("--precision-form", type=int, default=0, metavar="QUBITS",
                     help="also run the precision form (fusion to 6-qubit GEMMs on matrix cores) at this size")
     ap.add_argument("--batch-circuits", type=int, default=512)
+    ap.add_argument("--learned-form", type=int, default=33, metavar="QUBITS",
+                    help="kicked-Ising circuit size for the learned-versus-heuristic comparison (0 = skip)")
     ap.add_argument("--calibration-passes", type=int, default=0,
                     help="extra in-place scale-by-1 passes (k_scale: one read and one write of the state), "
                          "the known-size kernel that hardware byte counters are scaled by")
@@ -253,6 +299,8 @@ def main() -> int:
         result_extra["batch"] = run_batch(args.batch_circuits)
     if args.precision_form:
         result_extra["precision_form"] = run_precision_form(args.precision_form, args.depth, args.seed)
+    if args.learned_form and os.path.exists(model_path) and not args.no_learned:
+        result_extra["learned_form"] = run_learned_form(args.learned_form, 20, args.seed, model_path)
 
     cpu = None
     if not args.no_cpu:

@@ -32,11 +32,16 @@ HIDDEN = 64
 DEFAULT_MODEL = os.path.join(os.path.dirname(__file__), "..", "..", "..", "share", "planner_model.json")
 
 
-def training_circuits(qubits: list, seed: int):
+def training_circuits(qubits: list, seed: int, ising_qubits: list = ()):
     """Random, QFT and kicked-Ising circuits of the given sizes (the families
-    the planner is judged on), as (name, n, gates)."""
+    the planner is judged on), plus kicked-Ising circuits at the larger
+    `ising_qubits` sizes, as (name, n, gates)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
+    for n in ising_qubits:
+        for steps in (3, 8, 20):
+            theta = float(rng.uniform(0.1, 1.5))
+            out.append((f"ising{n}s{steps}", n, library.kicked_ising(n, library.heavy_hex_edges(n), steps, theta)))
     for n in qubits:
         for d in (3, 5, 8, 12, 16, 20):
             for k in range(2):
@@ -61,7 +66,9 @@ def collect(circuits, variants=None, repeats: int = 2, log=print) -> tuple:
     for name, n, gates in circuits:
         ops = _native.OpArray(lower_gates(gates))
         if n not in states:
-            states[n] = StateVector(n, "single")
+            for other in states.values():  # one state at a time: circuits come grouped by size
+                other.close()
+            states = {n: StateVector(n, "single")}
         sv = states[n]
         sv.set_zero()
         sv.apply_ops(ops)
@@ -151,19 +158,22 @@ def install(model: dict) -> Mlp:
 def main() -> int:
     ap = argparse.ArgumentParser(description="collect stage timings on the GPU and train the planner's cost model")
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--qubits", default="26,27,28")
+    ap.add_argument("--qubits", default="26,27,28,29,30")
+    ap.add_argument("--ising-qubits", default="31,32,33", help="sizes of the extra kicked-Ising circuits")
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--data", default="", help="also save the collected rows (.npz)")
     args = ap.parse_args()
     qubits = [int(q) for q in args.qubits.split(",")]
+    ising_qubits = [int(q) for q in args.ising_qubits.split(",") if q]
     t = time.perf_counter()
-    X, y, labels = collect(training_circuits(qubits, args.seed), log=lambda m: print(m, file=sys.stderr, flush=True))
+    X, y, labels = collect(training_circuits(qubits, args.seed, ising_qubits),
+                           log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"[learned] {len(X)} stage rows in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if args.data:
         np.savez(args.data, X=X, y=y, labels=np.array(labels))
     model = train(X, y, epochs=args.epochs, seed=args.seed)
-    model["data"] = {"qubits": qubits, "rows": int(len(X)), "seed": args.seed,
+    model["data"] = {"qubits": qubits, "ising_qubits": ising_qubits, "rows": int(len(X)), "seed": args.seed,
                      "source": "per-stage kernel time, HIP events around each register-tile stage launch"}
     save(model, args.out)
     print(json.dumps(model["metrics"]), flush=True)
