@@ -70,9 +70,10 @@ typedef struct qvd_options {
 } qvd_options;
 
 /* Plan variants (blocking.rs's lookahead with a reduced capacity, a greedy
- * choice of the next tile, register slots for the most-used qubits first)
- * and the per-stage features the learned planner scores them by. */
-#define QVD_PLAN_VARIANTS 16
+ * choice of the next tile, register slots for the most-used qubits first,
+ * tiles chosen among regions of the circuit's coupling graph) and the
+ * per-stage features the learned planner scores them by. */
+#define QVD_PLAN_VARIANTS 32
 #define QVD_PLAN_FEATURES 24
 
 /* qvd_options.flags: time every kernel launch with HIP events on the

@@ -148,6 +148,19 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
             if (!is_local[q]) return false;
         return true;
     };
+    // Coupling graph of the circuit (qubits that share an op), for the
+    // region candidates of variant bit 4.
+    std::vector<std::vector<uint32_t>> adj;
+    if (p.variant & 16) {
+        adj.assign(n, {});
+        for (const Op &op : ops) {
+            const std::vector<uint32_t> qs = op.qubits();
+            for (uint32_t a : qs)
+                for (uint32_t b : qs)
+                    if (a != b && !contains(adj[a], b)) adj[a].push_back(b);
+        }
+        for (auto &nb : adj) std::sort(nb.begin(), nb.end());
+    }
     std::vector<char> done(ops.size(), 0);
     size_t remaining = ops.size();
     while (remaining > 0) {
@@ -250,6 +263,47 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
                 if (k > best) {
                     best = k;
                     next = cnd;
+                }
+            }
+        }
+        if (p.variant & 16) {
+            // Region candidates: for every seed qubit, the breadth-first
+            // region of the coupling graph that fills the free capacity (a
+            // window along a line, a patch of a lattice). The next tile is the
+            // candidate (with the lookahead's) under which the most pending
+            // ops run now, so the tiles sweep the circuit's light cones
+            // instead of chasing its earliest pending op.
+            auto runnable = [&](const std::vector<uint32_t> &tile) {
+                std::vector<char> loc(n, 0);
+                for (uint32_t q : tile) loc[q] = 1;
+                return extract(ops, done, [&](const Op &op) {
+                           if (op.diagonal) return true;
+                           if (op.targets.size() > p.max_stage_op) return false;
+                           for (uint32_t q : op.targets)
+                               if (!loc[q]) return false;
+                           return true;
+                       }).size();
+            };
+            std::vector<std::vector<uint32_t>> cands{next};
+            for (uint32_t seed = contig; seed < n; seed++) {
+                std::vector<uint32_t> region{seed};
+                for (size_t i = 0; i < region.size() && region.size() < full_cap; i++)
+                    for (uint32_t w : adj[region[i]]) {
+                        if (region.size() >= full_cap) break;
+                        if (w >= contig && !contains(region, w)) region.push_back(w);
+                    }
+                std::vector<uint32_t> cand = tile_of(region);
+                if (std::find(cands.begin(), cands.end(), cand) == cands.end()) cands.push_back(std::move(cand));
+            }
+            size_t best = 0;
+            bool found = false;
+            for (const auto &cnd : cands) {
+                if (cnd == local) continue;
+                const size_t k = runnable(cnd);
+                if (!found || k > best) {
+                    best = k;
+                    next = cnd;
+                    found = true;
                 }
             }
         }

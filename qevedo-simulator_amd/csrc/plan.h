@@ -51,12 +51,14 @@ struct PlanParams {
     // reduced by that many qubits; bit 2: the next tile is the candidate (four
     // lookahead capacities and the most-targeted qubits) that runs the most
     // ops; bit 3: register-tile programs give register slots to the most-used
-    // bits first. Every variant applies the same ops in a dependency-respecting
-    // order, so the amplitudes agree up to rounding.
+    // bits first; bit 4: the next tile is the region of the circuit's coupling
+    // graph (one per seed qubit) under which the most ops run next. Every
+    // variant applies the same ops in a dependency-respecting order, so the
+    // amplitudes agree up to rounding.
     uint32_t variant = 0;
 };
 
-constexpr uint32_t PLAN_VARIANTS = 16;
+constexpr uint32_t PLAN_VARIANTS = 32;
 
 // The heuristic plan: src/blocking.rs's extract / plan_swaps loop with the
 // lookahead's wanted set becoming the next stage's tile qubits.

@@ -31,7 +31,7 @@ EXPORTS = (
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
     "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits",
 )
-PLAN_VARIANTS, PLAN_FEATURES = 16, 24
+PLAN_VARIANTS, PLAN_FEATURES = 32, 24
 
 
 class NativeError(RuntimeError):
