@@ -654,8 +654,8 @@ uint32_t vhandler(const VInstr &v) {
         case V_U1L:
             require(v.slot < 6, "lane bit out of range");
             if (ctrl) return H_U1L_CTRL + v.slot;
-            if (v.form == FORM_UNIT_ROT && v.slot >= 4) return H_U1L_ROT + 4 * (v.slot - 4) + (v.aux & 3);
-            if (v.form == FORM_UNIT_DIAG || v.form == FORM_UNIT_ROT) return H_U1L_DIAG + v.slot;
+            if (v.form == FORM_UNIT_ROT) return H_U1L_ROT + 4 * v.slot + (v.aux & 3);
+            if (v.form == FORM_UNIT_DIAG) return H_U1L_DIAG + v.slot;
             if (v.form == FORM_UNIT_ANTI) return H_U1L_ANTI + v.slot;
             return H_U1L_FULL + v.slot;
         case V_DIAG: return H_DIAG + 2 * (v.threaded ? 1 : 0) + (v.form == 1 ? 1 : 0);

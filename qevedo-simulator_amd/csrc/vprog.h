@@ -54,19 +54,19 @@ enum VHandler : uint32_t {
     H_U1R_ANTI = 20,  // + register bit
     H_U1R_FULL = 24,  // + register bit
     H_U1R_CTRL = 28,  // + register bit                  (full form, controls)
-    H_U1L_ROT = 32,   // + 4 * (lane bit - 4) + k        (lane bits 4 and 5)
-    H_U1L_DIAG = 40,  // + lane bit                      (also FORM_UNIT_ROT on lane bits 0-3)
-    H_U1L_ANTI = 46,  // + lane bit
-    H_U1L_FULL = 52,  // + lane bit
-    H_U1L_CTRL = 58,  // + lane bit                      (full form, controls)
-    H_DIAG = 64,      // + 2 * threaded + (form == 1)
-    H_SIGNS = 68,
-    H_PHASES = 69,
-    H_SCALE1 = 70,    // SCALE form 1: one uniform factor
-    H_SCALE = 71,
-    H_XCHG = 72,
-    H_LDSOP = 73,
-    H_COUNT = 74,
+    H_U1L_ROT = 32,   // + 4 * lane bit + k              (FORM_UNIT_ROT)
+    H_U1L_DIAG = 56,  // + lane bit
+    H_U1L_ANTI = 62,  // + lane bit
+    H_U1L_FULL = 68,  // + lane bit
+    H_U1L_CTRL = 74,  // + lane bit                      (full form, controls)
+    H_DIAG = 80,      // + 2 * threaded + (form == 1)
+    H_SIGNS = 84,
+    H_PHASES = 85,
+    H_SCALE1 = 86,    // SCALE form 1: one uniform factor
+    H_SCALE = 87,
+    H_XCHG = 88,
+    H_LDSOP = 89,
+    H_COUNT = 90,
 };
 
 // Forms of a 1-qubit gate's coefficients c[0..3]:

@@ -74,8 +74,8 @@ __device__ __forceinline__ T2 cdot(const T2 w0, const T2 x0, const T2 w1, const 
 
 // Value of v in lane ^ (1 << B): DPP quad permutes for bits 0 and 1, a
 // half-row mirror followed by a quad reversal for bit 2, a row rotation by 8
-// for bit 3, and the row-swap instructions for bits 4 and 5. All 64 lanes are
-// active wherever this is called.
+// for bit 3 and a swizzle for bit 4 (bit 5 uses the row swap below). All 64
+// lanes are active wherever this is called.
 template <int B>
 __device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane) {
     if constexpr (B == 0) {
@@ -85,9 +85,14 @@ __device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane) {
     } else if constexpr (B == 2) {
         const int m = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror: 7 - i
         return (uint32_t)__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false);      // quad_perm [3,2,1,0]
-    } else {
-        static_assert(B == 3, "lane bits 4 and 5 use row_swap");
+    } else if constexpr (B == 3) {
         return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    } else {
+        // Bit 4 crosses DPP rows: the LDS crossbar's swizzle in bit mode
+        // (lane ^ 16 within each half-wave; and-mask 0x1f, xor-mask 16) moves
+        // it on the LDS pipe, off the VALU.
+        static_assert(B == 4, "lane bit 5 uses the row swap");
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
     }
 }
 
@@ -307,6 +312,33 @@ __device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const T2 *c, uint32_t 
         else o = cdot(cs, a, cp, p);
         if (CTRL) o = sel(tp != 0, o, a);
         x[j] = o;
+    }
+}
+
+// FORM_UNIT_ROT on lane bit B < 5: the lane with bit B clear computes
+// a + i^K b and its partner b + i^(2-K) a, so each lane adds i^K (K odd) or
+// +-1 (K even, the sign set by the lane's bit) times its partner's amplitude:
+// two adds or FMAs per amplitude besides the partner fetch.
+template <typename T2, int RB, int B, int K>
+__device__ __forceinline__ void u1l_bit_rot(T2 (&x)[1 << RB], uint32_t lane) {
+    using R = decltype(T2::x);
+    const bool hi = (lane >> B) & 1;
+    const R sgn = (K == 0) == hi ? R(-1) : R(1);  // K even: +1 where the own term leads, -1 where it trails
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) {
+        T2 a = x[j];
+        const T2 p = lane_xor2<B>(a, lane);
+        if constexpr (K == 1) {
+            a.x = a.x - p.y;
+            a.y = a.y + p.x;
+        } else if constexpr (K == 3) {
+            a.x = a.x + p.y;
+            a.y = a.y - p.x;
+        } else {
+            a.x = fma_(sgn, p.x, a.x);
+            a.y = fma_(sgn, p.y, a.y);
+        }
+        x[j] = a;
     }
 }
 
@@ -636,8 +668,9 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 inline_coefs(v, c);
                 u1r<T2, RB, B, FORM_FULL, true>(x, c, (tid & v.tmask) == v.tval, v.rmask, v.rval);
             } else if constexpr (H < H_U1L_DIAG) {
-                constexpr int B = 4 + (H - H_U1L_ROT) / 4, K = (H - H_U1L_ROT) % 4;
-                u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K>(x, nullptr, lane, 1, 0, 0);
+                constexpr int B = (H - H_U1L_ROT) / 4, K = (H - H_U1L_ROT) % 4;
+                if constexpr (B < 5) u1l_bit_rot<T2, RB, B, K>(x, lane);
+                else u1l_pair<T2, RB, FORM_UNIT_ROT, false, B, K>(x, nullptr, lane, 1, 0, 0);
             } else if constexpr (H < H_DIAG) {
                 constexpr bool CTRL = H >= H_U1L_CTRL;
                 constexpr int F = CTRL ? FORM_FULL
@@ -651,7 +684,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                     rmask = v.rmask;
                     rval = v.rval;
                 }
-                if constexpr (B < 4) u1l_bit<T2, RB, F, CTRL, B>(x, c, lane, tp, rmask, rval);
+                if constexpr (B < 5) u1l_bit<T2, RB, F, CTRL, B>(x, c, lane, tp, rmask, rval);
                 else u1l_pair<T2, RB, F, CTRL, B>(x, c, lane, tp, rmask, rval);
             } else if constexpr (H < H_SIGNS) {
                 constexpr bool THREADED = ((H - H_DIAG) >> 1) != 0, SIGN = ((H - H_DIAG) & 1) != 0;
@@ -709,7 +742,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 }
             }
         };
-        static_assert(H_COUNT == 74, "the switch below lists every handler");
+        static_assert(H_COUNT == 90, "the switch below lists every handler");
 #define QVD_H(n) \
     case (n): handle(std::integral_constant<uint32_t, (n)>{}, v); break;
 #define QVD_H4(n) QVD_H(n) QVD_H((n) + 1) QVD_H((n) + 2) QVD_H((n) + 3)
@@ -720,7 +753,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
             switch ((w >> VH_SHIFT) & 0xffu) {
                 QVD_H4(0) QVD_H4(4) QVD_H4(8) QVD_H4(12) QVD_H4(16) QVD_H4(20) QVD_H4(24) QVD_H4(28)
                 QVD_H4(32) QVD_H4(36) QVD_H4(40) QVD_H4(44) QVD_H4(48) QVD_H4(52) QVD_H4(56) QVD_H4(60)
-                QVD_H4(64) QVD_H4(68) QVD_H(72) QVD_H(73)
+                QVD_H4(64) QVD_H4(68) QVD_H4(72) QVD_H4(76) QVD_H4(80) QVD_H4(84) QVD_H(88) QVD_H(89)
                 default: break;
             }
         }
