@@ -2,17 +2,21 @@
 //
 // One workgroup of W = 64 << WB threads owns a tile of 2^L amplitudes,
 // L = 6 + WB + RB, held in VGPRs: thread t keeps R = 2^RB amplitudes x[0..R).
-// Lane bits are tile bits 0..5 (physical qubits 0..5: each wave-wide load or
-// store is 64 consecutive amplitudes); the wave and register bits hold the
-// other tile bits in a role map that LDS exchanges change. The workgroup
-// interprets the stage's program (vprog.h) on each of its tiles and
-// prefetches the next tile into a second register set while it computes.
+// The lowest FB lane bits are tile bits 0..FB-1 (physical qubits 0..FB-1, so
+// every load and store is a run of 2^FB consecutive amplitudes); the other
+// lane bits, the wave bits and the register bits hold the other tile bits in
+// a role map that LDS exchanges change. The workgroup interprets the stage's
+// program (vprog.h) on each of its tiles: one switch per instruction on the
+// handler the host resolved it to. The complex128 geometry (2 waves per
+// SIMD) prefetches the next tile into a second register set; the complex64
+// one (4 waves per SIMD) relies on the other waves to cover a tile's loads.
 //
 // Coefficients indexed per lane (diagonal tables, scale tables, LDS-op
 // matrices) are copied to LDS once per workgroup; per-instruction uniform
-// coefficients are read through the scalar cache. Inside the tile loop no
-// vector memory access other than the tile's own loads and stores is issued,
-// so waiting on a coefficient never waits on the prefetch.
+// coefficients are read through the scalar cache (a 1-qubit gate's from its
+// instruction). Inside the tile loop no vector memory access other than the
+// tile's own loads and stores is issued, so no wait on a coefficient or an
+// instruction field waits on the tile's memory traffic.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
