@@ -83,7 +83,8 @@ struct qvd_sv {
     uint64_t len = 0;
     DBuf amps;
     hipStream_t stream = nullptr;
-    DBuf scratch;  // small reduction buffers
+    DBuf scratch;    // small reduction buffers
+    DBuf sample_ws;  // sampling workspace, grown to the largest call and kept
     double last_sample_seconds = 0;
     std::vector<double> stage_seconds;  // register-tile stage kernels of the last timed apply_ops
 
@@ -1271,24 +1272,36 @@ int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, ui
         const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
         const uint64_t nc = (sv->len + chunk - 1) / chunk;
         SampleScratch w;
-        DBuf sums(nc * sizeof(double)), prefix((nc + 1) * sizeof(double)), bounds((nc + 1) * sizeof(uint64_t));
-        DBuf tu(shots * sizeof(double)), ts(shots * sizeof(double)), du(shots * sizeof(double));
-        DBuf dout(shots * sizeof(uint64_t));
         w.sort_temp_bytes = sort_temp_bytes(shots);
-        DBuf temp(std::max<size_t>(w.sort_temp_bytes, 16));
-        w.sums = sums.as<double>();
-        w.prefix = prefix.as<double>();
-        w.bounds = bounds.as<uint64_t>();
-        w.targets_unsorted = tu.as<double>();
-        w.targets = ts.as<double>();
-        w.sort_temp = temp.p;
-        QVD_HIP(hipMemcpyAsync(du.p, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
         require(w.sort_temp_bytes > 0, "radix sort temporary size query failed");
+        // One workspace per handle, carved into the sampler's buffers (256-byte
+        // aligned) and kept for the next call: no allocation on repeated calls.
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            const size_t at = off;
+            off += (bytes + 255) & ~size_t(255);
+            return at;
+        };
+        const size_t o_sums = carve(nc * sizeof(double)), o_prefix = carve((nc + 1) * sizeof(double));
+        const size_t o_bounds = carve((nc + 1) * sizeof(uint64_t)), o_tu = carve(shots * sizeof(double));
+        const size_t o_ts = carve(shots * sizeof(double)), o_du = carve(shots * sizeof(double));
+        const size_t o_out = carve(shots * sizeof(uint64_t)), o_temp = carve(w.sort_temp_bytes);
+        if (sv->sample_ws.bytes < off) sv->sample_ws.alloc(off);
+        char *base = sv->sample_ws.as<char>();
+        w.sums = reinterpret_cast<double *>(base + o_sums);
+        w.prefix = reinterpret_cast<double *>(base + o_prefix);
+        w.bounds = reinterpret_cast<uint64_t *>(base + o_bounds);
+        w.targets_unsorted = reinterpret_cast<double *>(base + o_tu);
+        w.targets = reinterpret_cast<double *>(base + o_ts);
+        w.sort_temp = base + o_temp;
+        double *du = reinterpret_cast<double *>(base + o_du);
+        uint64_t *dout = reinterpret_cast<uint64_t *>(base + o_out);
+        QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
         with_type(sv, [&](auto *s) {
-            QVD_HIP(launch_sample(s, sv->len, chunk, du.as<double>(), shots, w, dout.as<uint64_t>(), sv->stream));
+            QVD_HIP(launch_sample(s, sv->len, chunk, du, shots, w, dout, sv->stream));
             return 0;
         });
-        QVD_HIP(hipMemcpyAsync(out, dout.p, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
+        QVD_HIP(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
         sv->sync();
         sv->last_sample_seconds = now() - t0;
     });

@@ -551,11 +551,34 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const T2 *__restrict__ s, ui
     sums[c] = acc;
 }
 
+// The sequential prefix over chunk sums, in one lane; the loads run a batch
+// of 32 ahead of the adds so the chain waits on the adds, not on memory.
 __global__ void k_prefix(const double *__restrict__ sums, uint64_t num_chunks, double *__restrict__ prefix) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        constexpr int U = 32;
         double p = 0;
         prefix[0] = 0;
-        for (uint64_t c = 0; c < num_chunks; c++) {
+        uint64_t c = 0;
+        double nxt[U];
+        if (c + U <= num_chunks) {
+#pragma unroll
+            for (int j = 0; j < U; j++) nxt[j] = sums[c + j];
+        }
+        for (; c + U <= num_chunks; c += U) {
+            double cur[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) cur[j] = nxt[j];
+            if (c + 2 * U <= num_chunks) {
+#pragma unroll
+                for (int j = 0; j < U; j++) nxt[j] = sums[c + U + j];
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                p = p + cur[j];
+                prefix[c + j + 1] = p;
+            }
+        }
+        for (; c < num_chunks; c++) {
             p = p + sums[c];
             prefix[c + 1] = p;
         }
@@ -601,7 +624,28 @@ __global__ __launch_bounds__(256) void k_walk(const T2 *__restrict__ s, uint64_t
     const uint64_t end = start + chunk < len ? start + chunk : len;
     uint64_t last_nonzero = start;
     double next = targets[shot];
-    for (uint64_t i = start; i < end && shot < hi; i++) {
+    // Amplitudes are loaded U at a time (the loads of a batch fly together),
+    // then walked one by one in index order.
+    constexpr int U = 8;
+    uint64_t i = start;
+    for (; i + U <= end && shot < hi; i += U) {
+        T2 v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) v[j] = s[i + j];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const double p = prob(v[j]);
+            if (p == 0.0) continue;
+            last_nonzero = i + j;
+            cumulative += p;
+            while (shot < hi && next < cumulative) {
+                out[shot] = i + j;
+                shot++;
+                if (shot < hi) next = targets[shot];
+            }
+        }
+    }
+    for (; i < end && shot < hi; i++) {
         const double p = prob(s[i]);
         if (p == 0.0) continue;
         last_nonzero = i;
