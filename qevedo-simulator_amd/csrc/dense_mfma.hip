@@ -6,10 +6,12 @@
 // U = Ur + i Ui and a column a = ar + i ai, the output is
 //     [or; oi] = R [ar; ai],   R = [[Ur, -Ui], [Ui, Ur]]   (2D x 2D real),
 // so a block of NC columns is the real product (2D x 2D) x (2D x NC).
-// A workgroup of 4 waves stages each block's amplitudes through LDS
-// (coalesced loads and stores, the next block's loads in flight during the
-// products); each wave keeps its 2D/4 rows of R in registers for the whole
-// launch and computes those rows of the output for all NC columns.
+// A workgroup of WAVES waves (4 for K = 5, 8 for K = 6: one 16-row tile of R
+// each, 108 VGPRs, so two workgroups share a CU at 4 waves per SIMD) stages
+// each block's amplitudes through LDS (coalesced loads and stores, the next
+// block's loads in flight during the products); each wave keeps its rows of
+// R in registers for the whole launch and computes those rows of the output
+// for all NC columns. The grid is 4 workgroups per CU.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,7 +24,6 @@ namespace qvd {
 namespace {
 
 constexpr int NC = 64;         // columns per block
-constexpr int WAVES = 4;       // 16 columns each
 constexpr int PAD = NC + 1;    // row pitch of the staged block (floats)
 
 __device__ __forceinline__ uint64_t spread(uint64_t x, const uint32_t *pos, uint32_t m) {
@@ -34,7 +35,7 @@ __device__ __forceinline__ uint64_t spread(uint64_t x, const uint32_t *pos, uint
     return x;
 }
 
-template <int K>
+template <int K, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_dense_mfma(float2 *__restrict__ s, const GateDesc d,
                                                            const float2 *__restrict__ mat) {
     constexpr int D = 1 << K, M = 2 * D;       // R is M x M
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_dense_mfma(float2 *__restrict__ 
     }
 }
 
-template <int K>
+template <int K, int WAVES>
 hipError_t launch_k(float2 *s, const GateDesc &d, const float2 *mat, unsigned grid, hipStream_t st) {
     constexpr int D = 1 << K;
     const size_t lds = (size_t)(2 * D * PAD) * sizeof(float) + (2 * NC + D) * sizeof(uint64_t);
@@ -170,11 +171,11 @@ hipError_t launch_k(float2 *s, const GateDesc &d, const float2 *mat, unsigned gr
     if (e != hipSuccess) return e;
     const uint64_t bit = uint64_t(1) << (dev & 63);
     if (!(attr_set.load() & bit)) {
-        e = hipFuncSetAttribute((const void *)k_dense_mfma<K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        e = hipFuncSetAttribute((const void *)k_dense_mfma<K, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attr_set.fetch_or(bit);
     }
-    k_dense_mfma<K><<<grid, 64 * WAVES, lds, st>>>(s, d, mat);
+    k_dense_mfma<K, WAVES><<<grid, 64 * WAVES, lds, st>>>(s, d, mat);
     return hipGetLastError();
 }
 
@@ -182,9 +183,9 @@ hipError_t launch_k(float2 *s, const GateDesc &d, const float2 *mat, unsigned gr
 
 hipError_t launch_dense_mfma(float2 *s, const GateDesc &d, const float2 *mat, unsigned cus, hipStream_t st) {
     const uint64_t blocks = (d.iterations + NC - 1) / NC;
-    const unsigned grid = (unsigned)std::min<uint64_t>(blocks, (uint64_t)cus * 3);
-    if (d.k == 5) return launch_k<5>(s, d, mat, grid, st);
-    if (d.k == 6) return launch_k<6>(s, d, mat, grid, st);
+    const unsigned grid = (unsigned)std::min<uint64_t>(blocks, (uint64_t)cus * 4);
+    if (d.k == 5) return launch_k<5, 4>(s, d, mat, grid, st);
+    if (d.k == 6) return launch_k<6, 8>(s, d, mat, grid, st);
     return hipErrorInvalidValue;
 }
 
