@@ -167,6 +167,30 @@ template <typename T2> void inline_coefficients(std::vector<VInstr> &ins, const 
     }
 }
 
+// Pair-phase runs (V_PHASES) read (a, b, w) per pair; the program keeps
+// (a, b, coefficient index) triples. This appends each run's pairs as
+// records (a, b, then w's words in the state's precision) to the integer
+// table and points the instruction at them (inl[0]), so the kernel loads a
+// pair in one scalar fetch with no dependent coefficient load.
+template <typename T2>
+void inline_phase_records(std::vector<VInstr> &ins, std::vector<uint32_t> &ints, const std::vector<T2> &coefs) {
+    constexpr uint32_t W = sizeof(T2) / 4;
+    for (VInstr &v : ins) {
+        if ((v.op & VOP_MASK) != V_PHASES) continue;
+        const uint32_t off = (uint32_t)ints.size();
+        for (uint32_t i = 0; i < v.k; i++) {
+            const uint32_t a = ints[v.aux + 3 * i], b = ints[v.aux + 3 * i + 1], ci = ints[v.aux + 3 * i + 2];
+            require(ci < coefs.size(), "internal: pair phase index out of range");
+            uint32_t w[W];
+            std::memcpy(w, &coefs[ci], sizeof(T2));
+            ints.push_back(a);
+            ints.push_back(b);
+            ints.insert(ints.end(), w, w + W);
+        }
+        v.inl[0] = off;
+    }
+}
+
 // One op as its own HBM pass: the dense kernel over the controlled subspace,
 // or the diagonal kernel.
 int g_num_cus = 0;
@@ -386,6 +410,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
     inline_coefficients(vins, vco_t);
+    inline_phase_records(vins, vints, vco_t);
     DBuf dvins(std::max<size_t>(vins.size(), 1) * sizeof(VInstr));
     DBuf dvco(std::max<size_t>(vco_t.size(), 1) * sizeof(T2));
     DBuf dvlds(std::max<size_t>(vlds.size(), 1) * sizeof(StageOp));
@@ -859,6 +884,7 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
     inline_coefficients(vins, vco_t);
+    inline_phase_records(vins, vints, vco_t);
     auto upload = [&](DBuf &buf, const void *src, size_t bytes) {
         buf.alloc(std::max<size_t>(bytes, 16));
         if (bytes) QVD_HIP(hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice));

@@ -410,10 +410,23 @@ __device__ __forceinline__ void phases(T2 (&x)[1 << RB], const QVD_CONST VInstr 
     T2 pr[RB];
 #pragma unroll
     for (int b = 0; b < RB; b++) pr[b] = one;
-    const uint32_t k = v.k, off = v.aux;
+    // Pair records (a, b, w in the state's precision), built at upload: one
+    // scalar fetch per pair, none depending on another load.
+    constexpr uint32_t RW = 2 + sizeof(T2) / 4;
+    const uint32_t k = v.k;
+    const QVD_CONST uint32_t *rec = ints + v.inl[0];
+#pragma unroll 4
     for (uint32_t i = 0; i < k; i++) {
-        const uint32_t a = ints[off + 3 * i], b = ints[off + 3 * i + 1];
-        const T2 w = coef[ints[off + 3 * i + 2]];
+        const QVD_CONST uint32_t *r = rec + RW * i;
+        const uint32_t a = r[0], b = r[1];
+        T2 w;
+        if constexpr (sizeof(T2) == 8) {
+            w.x = __uint_as_float(r[2]);
+            w.y = __uint_as_float(r[3]);
+        } else {
+            w.x = join64(r[2], r[3]);
+            w.y = join64(r[4], r[5]);
+        }
         const uint32_t xb = b & 0xffffu;
         const bool vb = (b & VT_KIND) == VT_THREAD ? ((tid >> xb) & 1) != 0 : ((base >> xb) & 1) != 0;
         const uint32_t xa = a & 0xffffu;
