@@ -531,6 +531,18 @@ __device__ __forceinline__ void lds_dense(T2 *__restrict__ tile, const T2 *__res
 
 // A 1-qubit gate's four coefficients, read from the instruction (inline
 // copy in the state's precision).
+// Workgroup barrier for the LDS tile: the wave's LDS accesses have completed
+// (lgkmcnt 0), then every wave meets. Unlike __syncthreads it does not wait
+// on global memory, so the tile's stores and prefetched loads stay in
+// flight; the empty asm statements keep the compiler from moving memory
+// accesses across.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // gfx9 encoding: lgkmcnt(0), vmcnt and expcnt not waited
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <typename T2>
 __device__ __forceinline__ void inline_coefs(const QVD_CONST VInstr &v, T2 (&c)[4]) {
     const QVD_CONST T2 *p = (const QVD_CONST T2 *)v.inl;
@@ -724,7 +736,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
 #pragma unroll
                 for (int j = 0; j < R; j++) x[j] = cm(cm(ccoef[v.coef + W + j], ft), x[j]);
             } else if constexpr (H == H_XCHG || H == H_LDSOP) {
-                __syncthreads();
+                lds_barrier();
 #pragma unroll
                 for (int j = 0; j < R; j++) {
                     uint32_t o = 0;
@@ -733,7 +745,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                         if ((j >> b) & 1) o |= 1u << rm[b];
                     tile[ti | o] = x[j];
                 }
-                __syncthreads();
+                lds_barrier();
                 if constexpr (H == H_LDSOP) {
                     const QVD_CONST StageOp &op = cops[v.aux];
                     const T2 *m = cbase + op.coef_off;
@@ -743,7 +755,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                         case 4: lds_dense<T2, 4, W>(tile, m, op, L); break;
                         default: lds_dense<T2, 1, W>(tile, m, op, L); break;
                     }
-                    __syncthreads();
+                    lds_barrier();
                 } else {
 #pragma unroll
                     for (int b = 0; b < RB; b++) rm[b] = cbyte(v.rmap, b);
