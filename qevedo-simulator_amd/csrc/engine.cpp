@@ -547,14 +547,14 @@ std::vector<float> plan_features(const std::vector<Step> &steps, uint32_t n, int
             switch (op) {
                 case V_U1R:
                     if (ctrl || v.form == FORM_FULL) f[5]++;
-                    else if (v.form == FORM_UNIT_ROT) f[2]++;
+                    else if (v.form == FORM_UNIT_ROT || v.form == FORM_UNIT_IMAG || v.form == FORM_UNIT_REAL) f[2]++;
                     else if (v.form == FORM_UNIT_DIAG) f[3]++;
                     else f[4]++;
                     break;
                 case V_U1L:
                     if (ctrl) f[9]++;
                     else if (v.slot < 4) f[6]++;
-                    else if (v.form == FORM_UNIT_ROT) f[7]++;
+                    else if (v.form == FORM_UNIT_ROT || v.form == FORM_UNIT_IMAG || v.form == FORM_UNIT_REAL) f[7]++;
                     else f[8]++;
                     break;
                 case V_DIAG:

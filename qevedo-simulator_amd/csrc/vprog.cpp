@@ -300,6 +300,13 @@ struct Gen {
                 // add and an FMA by 2 per part, in place.
                 emit_u1(q, {C64(1, 0), a, b, C64(1, 0)}, FORM_UNIT_ROT, {});
                 out.back().aux = (uint32_t)ka;
+            } else if (std::abs(a.real()) <= 1e-15 * std::abs(a) && std::abs(b.real()) <= 1e-15 * std::abs(b)) {
+                // Purely imaginary off-diagonals (RX-like; an exact zero real
+                // part, or one below 1e-15 of the value, which is dropped).
+                emit_u1(q, {C64(1, 0), C64(0, a.imag()), C64(0, b.imag()), C64(1, 0)}, FORM_UNIT_IMAG, {});
+            } else if (std::abs(a.imag()) <= 1e-15 * std::abs(a) && std::abs(b.imag()) <= 1e-15 * std::abs(b)) {
+                // Real off-diagonals (RY-like).
+                emit_u1(q, {C64(1, 0), C64(a.real(), 0), C64(b.real(), 0), C64(1, 0)}, FORM_UNIT_REAL, {});
             } else {
                 emit_u1(q, {C64(1, 0), a, b, C64(1, 0)}, FORM_UNIT_DIAG, {});
             }
@@ -649,6 +656,8 @@ uint32_t vhandler(const VInstr &v) {
             if (ctrl) return H_U1R_CTRL + v.slot;
             if (v.form == FORM_UNIT_ROT) return H_U1R_ROT + 4 * v.slot + (v.aux & 3);
             if (v.form == FORM_UNIT_DIAG) return H_U1R_DIAG + v.slot;
+            if (v.form == FORM_UNIT_IMAG) return H_U1R_IMAG + v.slot;
+            if (v.form == FORM_UNIT_REAL) return H_U1R_REAL + v.slot;
             if (v.form == FORM_UNIT_ANTI) return H_U1R_ANTI + v.slot;
             return H_U1R_FULL + v.slot;
         case V_U1L:
@@ -656,6 +665,8 @@ uint32_t vhandler(const VInstr &v) {
             if (ctrl) return H_U1L_CTRL + v.slot;
             if (v.form == FORM_UNIT_ROT) return H_U1L_ROT + 4 * v.slot + (v.aux & 3);
             if (v.form == FORM_UNIT_DIAG) return H_U1L_DIAG + v.slot;
+            if (v.form == FORM_UNIT_IMAG) return H_U1L_IMAG + v.slot;
+            if (v.form == FORM_UNIT_REAL) return H_U1L_REAL + v.slot;
             if (v.form == FORM_UNIT_ANTI) return H_U1L_ANTI + v.slot;
             return H_U1L_FULL + v.slot;
         case V_DIAG: return H_DIAG + 2 * (v.threaded ? 1 : 0) + (v.form == 1 ? 1 : 0);

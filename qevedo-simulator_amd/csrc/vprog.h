@@ -66,7 +66,11 @@ enum VHandler : uint32_t {
     H_SCALE = 87,
     H_XCHG = 88,
     H_LDSOP = 89,
-    H_COUNT = 90,
+    H_U1R_IMAG = 90,  // + register bit                  (FORM_UNIT_IMAG)
+    H_U1R_REAL = 94,  // + register bit                  (FORM_UNIT_REAL)
+    H_U1L_IMAG = 98,  // + lane bit
+    H_U1L_REAL = 104, // + lane bit
+    H_COUNT = 110,
 };
 
 // Forms of a 1-qubit gate's coefficients c[0..3]:
@@ -74,7 +78,18 @@ enum VHandler : uint32_t {
 //   FORM_UNIT_DIAG [[1, c1], [c2, 1]]   (4 fused multiply-adds per amplitude)
 //   FORM_UNIT_ANTI [[c0, 1], [1, c3]]
 //   FORM_UNIT_ROT  [[1, i^k], [i^(2-k), 1]]  (an add and an FMA by 2 per part; k = aux & 3)
-enum VForm : uint32_t { FORM_FULL = 0, FORM_UNIT_DIAG = 1, FORM_UNIT_ANTI = 2, FORM_UNIT_ROT = 3 };
+//   FORM_UNIT_IMAG [[1, i s1], [i s2, 1]]    (c1 = (0, s1), c2 = (0, s2): one FMA per part, as for
+//                                             RX-like gates)
+//   FORM_UNIT_REAL [[1, s1], [s2, 1]]        (c1 = (s1, 0), c2 = (s2, 0): one FMA per part, as for
+//                                             RY-like gates)
+enum VForm : uint32_t {
+    FORM_FULL = 0,
+    FORM_UNIT_DIAG = 1,
+    FORM_UNIT_ANTI = 2,
+    FORM_UNIT_ROT = 3,
+    FORM_UNIT_IMAG = 4,
+    FORM_UNIT_REAL = 5,
+};
 
 constexpr uint32_t VT_THREAD = 0u << 16;  // target bit is bit x of the thread index
 constexpr uint32_t VT_REG = 1u << 16;     // bit x of the register index

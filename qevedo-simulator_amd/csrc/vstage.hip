@@ -70,6 +70,16 @@ template <typename T2>
 __device__ __forceinline__ T2 cfma(const T2 w, const T2 x, const T2 a) {
     return mk<T2>(fma_(-w.y, x.y, fma_(w.x, x.x, a.x)), fma_(w.y, x.x, fma_(w.x, x.y, a.y)));
 }
+// a + i s x (s real)
+template <typename T2>
+__device__ __forceinline__ T2 imag_fma(const decltype(T2::x) s, const T2 x, const T2 a) {
+    return mk<T2>(fma_(-s, x.y, a.x), fma_(s, x.x, a.y));
+}
+// a + s x (s real)
+template <typename T2>
+__device__ __forceinline__ T2 real_fma(const decltype(T2::x) s, const T2 x, const T2 a) {
+    return mk<T2>(fma_(s, x.x, a.x), fma_(s, x.y, a.y));
+}
 // w0 * x0 + w1 * x1
 template <typename T2>
 __device__ __forceinline__ T2 cdot(const T2 w0, const T2 x0, const T2 w1, const T2 x1) {
@@ -282,6 +292,12 @@ __device__ __forceinline__ void u1r(T2 (&x)[1 << RB], const T2 *c, uint32_t tpre
         if (FORM == FORM_UNIT_DIAG) {
             o0 = cfma(c1, b, a);
             o1 = cfma(c2, a, b);
+        } else if (FORM == FORM_UNIT_IMAG) {
+            o0 = imag_fma(c1.y, b, a);
+            o1 = imag_fma(c2.y, a, b);
+        } else if (FORM == FORM_UNIT_REAL) {
+            o0 = real_fma(c1.x, b, a);
+            o1 = real_fma(c2.x, a, b);
         } else if (FORM == FORM_UNIT_ANTI) {
             o0 = cfma(c0, a, b);
             o1 = cfma(c3, b, a);
@@ -312,6 +328,8 @@ __device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const T2 *c, uint32_t 
         const T2 p = lane_xor2<B>(a, lane);
         T2 o;
         if (FORM == FORM_UNIT_DIAG) o = cfma(cp, p, a);
+        else if (FORM == FORM_UNIT_IMAG) o = imag_fma(cp.y, p, a);
+        else if (FORM == FORM_UNIT_REAL) o = real_fma(cp.x, p, a);
         else if (FORM == FORM_UNIT_ANTI) o = cfma(cs, a, p);
         else o = cdot(cs, a, cp, p);
         if (CTRL) o = sel(tp != 0, o, a);
@@ -378,6 +396,12 @@ __device__ __forceinline__ void u1l_pair(T2 (&x)[1 << RB], const T2 *c, uint32_t
         if (FORM == FORM_UNIT_DIAG) {
             o0 = cfma(c1, b, a);
             o1 = cfma(c2, a, b);
+        } else if (FORM == FORM_UNIT_IMAG) {
+            o0 = imag_fma(c1.y, b, a);
+            o1 = imag_fma(c2.y, a, b);
+        } else if (FORM == FORM_UNIT_REAL) {
+            o0 = real_fma(c1.x, b, a);
+            o1 = real_fma(c2.x, a, b);
         } else if (FORM == FORM_UNIT_ANTI) {
             o0 = cfma(c0, a, b);
             o1 = cfma(c3, b, a);
@@ -682,7 +706,21 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         // rotations fixed at compile time.
         auto handle = [&](auto hc, const QVD_CONST VInstr &v) {
             constexpr uint32_t H = decltype(hc)::value;
-            if constexpr (H < H_U1R_DIAG) {
+            if constexpr (H >= H_U1R_IMAG) {
+                // Unit forms with real or imaginary off-diagonals.
+                T2 c[4];
+                inline_coefs(v, c);
+                if constexpr (H < H_U1L_IMAG) {
+                    constexpr int F = H < H_U1R_REAL ? FORM_UNIT_IMAG : FORM_UNIT_REAL;
+                    constexpr int B = (H - H_U1R_IMAG) % 4;
+                    u1r<T2, RB, B, F, false>(x, c, 1, 0, 0);
+                } else {
+                    constexpr int F = H < H_U1L_REAL ? FORM_UNIT_IMAG : FORM_UNIT_REAL;
+                    constexpr int B = (H - H_U1L_IMAG) % 6;
+                    if constexpr (B < 5) u1l_bit<T2, RB, F, false, B>(x, c, lane, 1, 0, 0);
+                    else u1l_pair<T2, RB, F, false, B>(x, c, lane, 1, 0, 0);
+                }
+            } else if constexpr (H < H_U1R_DIAG) {
                 constexpr int B = (H - H_U1R_ROT) / 4, K = (H - H_U1R_ROT) % 4;
                 u1r<T2, RB, B, FORM_UNIT_ROT, false, K>(x, nullptr, 1, 0, 0);
             } else if constexpr (H < H_U1R_CTRL) {
@@ -771,7 +809,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 }
             }
         };
-        static_assert(H_COUNT == 90, "the switch below lists every handler");
+        static_assert(H_COUNT == 110, "the switch below lists every handler");
 #define QVD_H(n) \
     case (n): handle(std::integral_constant<uint32_t, (n)>{}, v); break;
 #define QVD_H4(n) QVD_H(n) QVD_H((n) + 1) QVD_H((n) + 2) QVD_H((n) + 3)
@@ -783,6 +821,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 QVD_H4(0) QVD_H4(4) QVD_H4(8) QVD_H4(12) QVD_H4(16) QVD_H4(20) QVD_H4(24) QVD_H4(28)
                 QVD_H4(32) QVD_H4(36) QVD_H4(40) QVD_H4(44) QVD_H4(48) QVD_H4(52) QVD_H4(56) QVD_H4(60)
                 QVD_H4(64) QVD_H4(68) QVD_H4(72) QVD_H4(76) QVD_H4(80) QVD_H4(84) QVD_H(88) QVD_H(89)
+                QVD_H4(90) QVD_H4(94) QVD_H4(98) QVD_H4(102) QVD_H4(106)
                 default: break;
             }
         }

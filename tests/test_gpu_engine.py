@@ -175,7 +175,16 @@ def test_register_tiles_structured_circuits(precision):
     random circuit family on register tiles, against the FP64 oracle."""
     _note(f"structured circuits {precision}")
     n = 16
+    # Layers of RX and RY on every qubit (off-diagonals purely imaginary or
+    # real after the unit-form split) between CZ ladders, so the real and
+    # imaginary unit forms run on every register and lane bit.
+    rxry = []
+    for layer in range(6):
+        for q in range(n):
+            rxry.append(("rx" if (q + layer) % 2 else "ry", (q,), (0.3 + 0.17 * q + 0.5 * layer,)))
+        rxry += [("cz", (q, q + 1), ()) for q in range(layer % 2, n - 1, 2)]
     circuits = [
+        rxry,
         library.kicked_ising(n, library.heavy_hex_edges(n), 5, 0.9),
         library.hardware_efficient_ansatz(n, 3, 4),
         library.random_circuit(n, 14, 11),
