@@ -101,36 +101,41 @@ constexpr uint32_t VMAX_R_BITS = 6;
 constexpr uint32_t VMAX_L_FLEX = 4;  // flexible lane bits
 constexpr uint32_t VMAX_REGS = 32;
 
-struct VInstr {
+// The fields the common instructions read (the op word, counts, offsets, a
+// sign run's mask and a complex64 gate's coefficients) fill the first
+// 64 bytes, and the struct is 64-byte aligned, so interpreting such an
+// instruction touches one scalar-cache line: a long stage program keeps its
+// hot lines in the scalar cache instead of streaming whole records.
+struct alignas(64) VInstr {
     uint32_t op;    // VOp | VF_* flags
+    uint32_t k;     // DIAG: number of targets; SCALE: number of non-tile factors; SIGNS / PHASES: pairs
+    uint32_t aux;   // LDSOP: index of its StageOp; SCALE: first non-tile qubit in the int array
+    uint32_t coef;  // first coefficient (complex) of the op
     uint32_t form;
     uint32_t slot;  // U1R: register bit; U1L: lane bit
-    uint32_t k;     // DIAG: number of targets; SCALE: number of non-tile factors
-    uint32_t tmask, tval;  // controls on thread-index bits
-    uint32_t rmask, rval;  // controls on register-index bits
-    uint64_t gmask, gval;  // controls on non-tile qubits (checked per tile)
-    uint64_t neg;          // DIAG, form 1: entries equal to -1 (the others are +1)
-    uint32_t coef;         // first coefficient (complex) of the op
-    uint32_t aux;          // LDSOP: index of its StageOp; SCALE: first non-tile qubit in the int array
+    uint64_t neg;   // SIGNS: per-register signs; DIAG, form 1: entries equal to -1 (the others are +1)
     // U1R / U1L: the four matrix coefficients in the state's precision (4 x
     // float2 or 4 x double2), written when the program is uploaded, so the
     // kernel reads them from the instruction instead of following coef.
+    // PHASES: inl[0] is the offset of its pair records in the int array.
     uint32_t inl[16];
+    uint32_t tmask, tval;  // controls on thread-index bits
+    uint32_t rmask, rval;  // controls on register-index bits
+    uint64_t gmask, gval;  // controls on non-tile qubits (checked per tile)
     uint32_t tgt[MAX_GATE_QUBITS];            // DIAG targets (VT_* | bit)
     uint32_t threaded;                        // DIAG: some target or control is a thread-index bit
     uint8_t rtab[VMAX_REGS];                  // DIAG: register-index part of the entry index, per register
     uint8_t lmap[VMAX_L_FLEX];                // XCHG: tile bit of each flexible lane bit
     uint8_t wmap[VMAX_W_BITS];                // XCHG: tile bit of each wave bit
     uint8_t rmap[VMAX_R_BITS];                // XCHG: tile bit of each register bit
-    uint8_t pad[2];
 };
 
 // The handler of an instruction (VF_* flags and fields already set); the
 // register-bit count must be at most 4.
 uint32_t vhandler(const VInstr &v);
 
-static_assert(offsetof(VInstr, coef) == 56 && offsetof(VInstr, aux) == 60 && offsetof(VInstr, inl) == 64,
-              "the kernel reads an instruction's first words by position");
+static_assert(offsetof(VInstr, neg) == 24 && offsetof(VInstr, inl) == 32 && sizeof(VInstr) % 64 == 0,
+              "a complex64 gate's fields and a sign run's mask sit in an instruction's first 64 bytes");
 
 // One register-tile stage as launched. A launch runs one or more of them
 // (distinct states of a batch), each on its own range of workgroups.
