@@ -563,7 +563,9 @@ std::vector<float> plan_features(const std::vector<Step> &steps, uint32_t n, int
                     break;
                 case V_SIGNS:
                     f[12]++;
-                    f[13] += v.k;
+                    // pairs through the generic loop, plus the masks evaluated
+                    f[13] += v.k + ((v.form & 1) ? 1 : 0);
+                    for (int t = 0; t < 10; t++) f[13] += v.inl[4 + t] != 0 ? 1 : 0;
                     if (v.neg) f[14]++;
                     break;
                 case V_PHASES:

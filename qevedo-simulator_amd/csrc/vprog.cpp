@@ -133,8 +133,28 @@ struct Gen {
         uint64_t rr = 0;
         std::vector<C64> rtable(R, C64(1, 0));
         bool has_rr = false;
+        uint32_t tr_mask[VMAX_R_BITS] = {}, tt_mask[10] = {};
+        bool has_tr = false, has_tt = false;
         for (auto &pp : prun) {
             uint32_t a = encode(pp.p), b = encode(pp.q);
+            if (sign && (a & VT_KIND) != VT_GLOBAL && (b & VT_KIND) != VT_GLOBAL &&
+                ((a & VT_KIND) == VT_THREAD || (b & VT_KIND) == VT_THREAD)) {
+                // Sign pairs without a non-tile bit become bit masks (the sum
+                // is mod 2, so a repeated pair cancels): a thread bit t and a
+                // register bit r flip register r's linear term where thread
+                // bit t is set (tr_mask[r]); two thread bits t < u add
+                // x_t x_u to the thread's parity (tt_mask[t]).
+                if ((a & VT_KIND) == VT_REG) std::swap(a, b);
+                const uint32_t xa = a & 0xffffu, xb = b & 0xffffu;
+                if ((b & VT_KIND) == VT_REG) {
+                    tr_mask[xb] ^= 1u << xa;
+                    has_tr = true;
+                } else {
+                    tt_mask[std::min(xa, xb)] ^= 1u << std::max(xa, xb);
+                    has_tt = true;
+                }
+                continue;
+            }
             if ((a & VT_KIND) == VT_REG && (b & VT_KIND) == VT_REG) {
                 const uint32_t ra = a & 0xffffu, rb = b & 0xffffu;
                 for (uint32_t j = 0; j < R; j++)
@@ -156,6 +176,10 @@ struct Gen {
         }
         if (sign) {
             v.neg = rr;
+            require(6 + g.w_bits <= 10 && g.r_bits <= 4, "internal: sign masks cover 10 thread and 4 register bits");
+            v.form = (has_tr ? 1u : 0u) | (has_tt ? 2u : 0u);
+            for (uint32_t r = 0; r < 4; r++) v.inl[r] = tr_mask[r];
+            for (uint32_t t = 0; t < 10; t++) v.inl[4 + t] = tt_mask[t];
         } else if (has_rr) {
             v.form = 1;
             v.coef = (uint32_t)coefs.size();

@@ -192,21 +192,37 @@ __device__ __forceinline__ double2 xor_sign(double2 a, uint32_t m) {
 
 // A run of CZ-like gates: amplitude (thread, register j) changes sign when
 // the quadratic form sum x_p x_q over the run's pairs is odd. Pairs of two
-// register bits arrive as the per-register mask v.neg; every other pair is
-// (encoded bit, encoded bit) in the int array. The parity of register j is
-// built as t[j] = t[j with its lowest set bit cleared] ^ M[that bit], one
-// exclusive-or per register, kept in bit 31.
+// register bits arrive as the per-register mask v.neg; pairs of a thread bit
+// and a register bit r as the thread-bit mask inl[r] (register r's linear
+// term is the parity of tid & inl[r]); pairs of thread bits t < u as the
+// mask inl[4 + t] (the thread's constant term gains x_t * parity(tid &
+// inl[4 + t])); pairs with a non-tile qubit are (encoded bit, encoded bit)
+// in the int array. The parity of register j is built as t[j] = t[j with its
+// lowest set bit cleared] ^ M[that bit], one exclusive-or per register, kept
+// in bit 31.
 template <typename T2, int RB>
 __device__ __forceinline__ void signs(T2 (&x)[1 << RB], const QVD_CONST VInstr &v, const QVD_CONST uint32_t *ints,
                                       uint32_t tid, uint64_t base) {
     const uint64_t neg = v.neg;
-    const uint32_t k = v.k;
-    if (k == 0) {
+    const uint32_t k = v.k, form = v.form;
+    if (k == 0 && form == 0) {
 #pragma unroll
         for (int j = 0; j < (1 << RB); j++) x[j] = xor_sign(x[j], (uint32_t)((neg >> j) & 1) << 31);
         return;
     }
     uint32_t al = 0, gam = 0, gm = 0;
+    if (form & 1) {
+#pragma unroll
+        for (int r = 0; r < RB; r++) gam |= (__builtin_popcount(tid & v.inl[r]) & 1u) << r;
+    }
+    if (form & 2) {
+#pragma unroll
+        for (int t = 0; t < 10; t++) {
+            const uint32_t m = v.inl[4 + t];
+            if (m != 0) al ^= (tid >> t) & __builtin_popcount(tid & m);
+        }
+        al &= 1u;
+    }
     for (uint32_t i = 0; i < k; i++) {
         const uint32_t a = ints[v.aux + 2 * i], b = ints[v.aux + 2 * i + 1];
         const uint32_t ka = a & VT_KIND, kb = b & VT_KIND, xa = a & 0xffffu, xb = b & 0xffffu;
