@@ -43,7 +43,7 @@ enum VOp : uint32_t {
 // fields only where used.
 constexpr uint32_t VOP_MASK = 0xffu;
 constexpr uint32_t VF_CTRL = 1u << 8;   // controls on thread or register bits (tmask / rmask nonzero)
-constexpr uint32_t VF_GCTRL = 1u << 9;  // controls on non-tile qubits (gmask nonzero)
+constexpr uint32_t VF_GCTRL = 1u << 31;  // controls on non-tile qubits (gmask nonzero): the sign bit, one compare
 constexpr uint32_t VH_SHIFT = 16;
 
 // Kernel code paths. Each fixes at compile time what the VOp, form, slot and

@@ -832,7 +832,9 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         for (const QVD_CONST VInstr *pv = ip; pv != ip_end; ++pv) {
             const QVD_CONST VInstr &v = *pv;
             const uint32_t w = v.op;
-            if ((w & VF_GCTRL) && (base & v.gmask) != v.gval) continue;
+            if (__builtin_expect((int32_t)w < 0, 0)) {  // VF_GCTRL is the sign bit
+                if ((base & v.gmask) != v.gval) continue;
+            }
             switch ((w >> VH_SHIFT) & 0xffu) {
                 QVD_H4(0) QVD_H4(4) QVD_H4(8) QVD_H4(12) QVD_H4(16) QVD_H4(20) QVD_H4(24) QVD_H4(28)
                 QVD_H4(32) QVD_H4(36) QVD_H4(40) QVD_H4(44) QVD_H4(48) QVD_H4(52) QVD_H4(56) QVD_H4(60)
