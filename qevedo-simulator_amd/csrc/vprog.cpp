@@ -666,7 +666,10 @@ VProgram build_vprogram(const Stage &stage, uint32_t n, const VGeom &g, PendingD
     for (VInstr &v : p.instr) {
         if (v.tmask != 0 || v.rmask != 0) v.op |= VF_CTRL;
         if (v.gmask != 0) v.op |= VF_GCTRL;
-        v.op |= vhandler(v) << VH_SHIFT;
+        const uint32_t h = vhandler(v);
+        require(v.gmask == 0 || (h >= H_U1R_FULL && h < H_U1L_ROT) || (h >= H_U1L_FULL && h < H_SIGNS) || h == H_LDSOP,
+                "internal: non-tile controls on an instruction whose kernel path does not check them");
+        v.op |= h << VH_SHIFT;
     }
     return p;
 }

@@ -722,6 +722,14 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         // rotations fixed at compile time.
         auto handle = [&](auto hc, const QVD_CONST VInstr &v) {
             constexpr uint32_t H = decltype(hc)::value;
+            // Controls on non-tile qubits come only with the full and
+            // controlled 1-qubit forms, general diagonals and LDS ops (the
+            // host checks this); those arms skip tiles where they do not
+            // hold, and no other arm spends a compare on them.
+            constexpr bool MAY_GCTRL = (H >= H_U1R_FULL && H < H_U1L_ROT) || (H >= H_U1L_FULL && H < H_SIGNS) ||
+                                       H == H_LDSOP;
+            if constexpr (MAY_GCTRL)
+                if ((int32_t)v.op < 0 && (base & v.gmask) != v.gval) return;  // VF_GCTRL is the sign bit
             if constexpr (H >= H_U1R_IMAG) {
                 // Unit forms with real or imaginary off-diagonals.
                 T2 c[4];
@@ -832,9 +840,6 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         for (const QVD_CONST VInstr *pv = ip; pv != ip_end; ++pv) {
             const QVD_CONST VInstr &v = *pv;
             const uint32_t w = v.op;
-            if (__builtin_expect((int32_t)w < 0, 0)) {  // VF_GCTRL is the sign bit
-                if ((base & v.gmask) != v.gval) continue;
-            }
             switch ((w >> VH_SHIFT) & 0xffu) {
                 QVD_H4(0) QVD_H4(4) QVD_H4(8) QVD_H4(12) QVD_H4(16) QVD_H4(20) QVD_H4(24) QVD_H4(28)
                 QVD_H4(32) QVD_H4(36) QVD_H4(40) QVD_H4(44) QVD_H4(48) QVD_H4(52) QVD_H4(56) QVD_H4(60)
