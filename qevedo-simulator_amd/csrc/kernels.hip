@@ -531,7 +531,7 @@ __global__ void k_collapse(T2 *__restrict__ s, uint64_t len, uint32_t qubit, uin
 // by one; so one lane owns one chunk. Loads are 16-byte vectors.
 // ---------------------------------------------------------------------------
 template <typename T2>
-__global__ __launch_bounds__(256) void k_chunk_sums(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
+__global__ __launch_bounds__(64) void k_chunk_sums(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
                                                     uint64_t num_chunks, double *__restrict__ sums) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= num_chunks) return;
@@ -539,7 +539,9 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const T2 *__restrict__ s, ui
     const uint64_t end = start + chunk < len ? start + chunk : len;
     double acc = 0;
     uint64_t i = start;
-    constexpr int U = 8;
+    // One lane per chunk and few chunks (2^15 at 30 qubits): each batch of
+    // loads is a memory round trip the lane waits for, so batches are long.
+    constexpr int U = 32;
     for (; i + U <= end; i += U) {
         T2 v[U];
 #pragma unroll
@@ -551,32 +553,40 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const T2 *__restrict__ s, ui
     sums[c] = acc;
 }
 
-// The sequential prefix over chunk sums, in one lane; the loads run a batch
-// of 32 ahead of the adds so the chain waits on the adds, not on memory.
+// The sequential prefix over chunk sums, in one lane. Two batches of U sums
+// alternate: one batch's loads fly while the other's adds run, each batch's
+// prefixes overwrite its own registers and are stored after its adds, so the
+// dependent add chain waits neither on memory nor on stores.
 __global__ void k_prefix(const double *__restrict__ sums, uint64_t num_chunks, double *__restrict__ prefix) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         constexpr int U = 32;
         double p = 0;
         prefix[0] = 0;
         uint64_t c = 0;
-        double nxt[U];
-        if (c + U <= num_chunks) {
+        double a[U], b[U];
+        auto load = [&](double(&dst)[U], uint64_t at) {
 #pragma unroll
-            for (int j = 0; j < U; j++) nxt[j] = sums[c + j];
-        }
-        for (; c + U <= num_chunks; c += U) {
-            double cur[U];
-#pragma unroll
-            for (int j = 0; j < U; j++) cur[j] = nxt[j];
-            if (c + 2 * U <= num_chunks) {
-#pragma unroll
-                for (int j = 0; j < U; j++) nxt[j] = sums[c + U + j];
-            }
+            for (int j = 0; j < U; j++) dst[j] = sums[at + j];
+        };
+        auto scan_store = [&](double(&buf)[U], uint64_t at) {
 #pragma unroll
             for (int j = 0; j < U; j++) {
-                p = p + cur[j];
-                prefix[c + j + 1] = p;
+                p = p + buf[j];
+                buf[j] = p;
             }
+#pragma unroll
+            for (int j = 0; j < U; j++) prefix[at + j + 1] = buf[j];
+        };
+        if (c + U <= num_chunks) load(a, c);
+        for (; c + 2 * U <= num_chunks; c += 2 * U) {
+            load(b, c + U);
+            scan_store(a, c);
+            if (c + 3 * U <= num_chunks) load(a, c + 2 * U);
+            scan_store(b, c + U);
+        }
+        if (c + U <= num_chunks) {  // a holds the last whole batch
+            scan_store(a, c);
+            c += U;
         }
         for (; c < num_chunks; c++) {
             p = p + sums[c];
@@ -609,7 +619,7 @@ __global__ void k_bounds(const double *__restrict__ prefix, uint64_t num_chunks,
 }
 
 template <typename T2>
-__global__ __launch_bounds__(256) void k_walk(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
+__global__ __launch_bounds__(64) void k_walk(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
                                               uint64_t num_chunks, const double *__restrict__ prefix,
                                               const uint64_t *__restrict__ bounds, const double *__restrict__ targets,
                                               uint64_t shots, uint64_t *__restrict__ out) {
@@ -625,8 +635,10 @@ __global__ __launch_bounds__(256) void k_walk(const T2 *__restrict__ s, uint64_t
     uint64_t last_nonzero = start;
     double next = targets[shot];
     // Amplitudes are loaded U at a time (the loads of a batch fly together),
-    // then walked one by one in index order.
-    constexpr int U = 8;
+    // then walked one by one in index order. (Two alternating batches, one
+    // in flight while the other is walked, measured slower: 5.3 ms against
+    // 3.6 ms for 10^6 shots at 30 qubits.)
+    constexpr int U = 32;
     uint64_t i = start;
     for (; i + U <= end && shot < hi; i += U) {
         T2 v[U];
@@ -784,7 +796,8 @@ template <typename T2>
 hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double *uniforms, uint64_t shots,
                          SampleScratch &w, uint64_t *out, hipStream_t st) {
     const uint64_t nc = (len + chunk - 1) / chunk;
-    k_chunk_sums<T2><<<grid_for(nc, 256, 1u << 30), 256, 0, st>>>(s, len, chunk, nc, w.sums);
+    // 64-lane workgroups for the per-chunk kernels, so 2^15 chunks spread over every CU.
+    k_chunk_sums<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.sums);
     k_prefix<<<1, 64, 0, st>>>(w.sums, nc, w.prefix);
     k_targets<<<grid_for(shots, 256), 256, 0, st>>>(uniforms, w.prefix + nc, shots, w.targets_unsorted);
     size_t bytes = w.sort_temp_bytes;
@@ -792,8 +805,8 @@ hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double
                                                      sizeof(double) * 8, st);
     if (e != hipSuccess) return e;
     k_bounds<<<grid_for(nc + 1, 256, 1u << 30), 256, 0, st>>>(w.prefix, nc, w.targets, shots, w.bounds);
-    k_walk<T2><<<grid_for(nc, 256, 1u << 30), 256, 0, st>>>(s, len, chunk, nc, w.prefix, w.bounds, w.targets,
-                                                            shots, out);
+    k_walk<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.prefix, w.bounds, w.targets,
+                                                          shots, out);
     return hipGetLastError();
 }
 
