@@ -635,10 +635,11 @@ __global__ __launch_bounds__(64) void k_walk(const T2 *__restrict__ s, uint64_t 
     uint64_t last_nonzero = start;
     double next = targets[shot];
     // Amplitudes are loaded U at a time (the loads of a batch fly together),
-    // then walked one by one in index order. (Two alternating batches, one
-    // in flight while the other is walked, measured slower: 5.3 ms against
-    // 3.6 ms for 10^6 shots at 30 qubits.)
-    constexpr int U = 32;
+    // then walked one by one in index order. (Two alternating batches of 32,
+    // one in flight while the other is walked, measured slower: 5.3 ms
+    // against 3.6 ms for one batch of 32 and 3.3 ms for one of 64, 10^6
+    // shots at 30 qubits.)
+    constexpr int U = 64;
     uint64_t i = start;
     for (; i + U <= end && shot < hi; i += U) {
         T2 v[U];
