@@ -89,18 +89,21 @@ __device__ __forceinline__ T2 cdot(const T2 w0, const T2 x0, const T2 w1, const 
 // Value of v in lane ^ (1 << B): DPP quad permutes for bits 0 and 1, a
 // half-row mirror followed by a quad reversal for bit 2, a row rotation by 8
 // for bit 3 and a swizzle for bit 4 (bit 5 uses the row swap below). All 64
-// lanes are active wherever this is called.
+// lanes are active wherever this is called, and every source lane lies in
+// the reading lane's row, so bound_ctrl changes no value; it lets the
+// compiler fold the move into a following add or subtract as its DPP operand
+// (the rotation form on lane bits 0-3 then costs one instruction per part).
 template <int B>
 __device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane) {
     if constexpr (B == 0) {
-        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
     } else if constexpr (B == 1) {
-        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
     } else if constexpr (B == 2) {
-        const int m = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror: 7 - i
-        return (uint32_t)__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, false);      // quad_perm [3,2,1,0]
+        const int m = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);  // row_half_mirror: 7 - i
+        return (uint32_t)__builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, true);      // quad_perm [3,2,1,0]
     } else if constexpr (B == 3) {
-        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, true);  // row_ror:8
     } else {
         // Bit 4 crosses DPP rows: the LDS crossbar's swizzle in bit mode
         // (lane ^ 16 within each half-wave; and-mask 0x1f, xor-mask 16) moves
