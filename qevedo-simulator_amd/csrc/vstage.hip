@@ -178,6 +178,22 @@ __device__ __forceinline__ double2 lane_xor2(double2 v, uint32_t lane) {
     return make_double2(lane_xor_f64<B>(v.x, lane), lane_xor_f64<B>(v.y, lane));
 }
 
+// Value of v in lane ^ (1 << B), B < 5, through the LDS crossbar's swizzle
+// (bit mode: and-mask 0x1f, xor-mask 1 << B within each 32-lane half): it
+// runs on the LDS pipe, so a gate whose FMAs keep the VALU busy does not also
+// spend VALU cycles on the partner fetch.
+template <int B>
+__device__ __forceinline__ float2 lane_xor_swz2(float2 v) {
+    static_assert(B < 5, "swizzle reaches lane bits 0-4");
+    constexpr int PAT = 0x1F | ((1 << B) << 10);
+    return make_float2(__int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v.x), PAT)),
+                       __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v.y), PAT)));
+}
+template <int B>
+__device__ __forceinline__ double2 lane_xor_swz2(double2 v) {
+    return lane_xor2<B>(v, 0u);
+}
+
 template <typename T2>
 __device__ __forceinline__ T2 sel(bool p, T2 a, T2 b) {
     return mk<T2>(p ? a.x : b.x, p ? a.y : b.y);
@@ -344,7 +360,10 @@ __device__ __forceinline__ void u1l_bit(T2 (&x)[1 << RB], const T2 *c, uint32_t 
     for (int j = 0; j < (1 << RB); j++) {
         if (CTRL && ((uint32_t)j & rmask) != rval) continue;
         const T2 a = x[j];
-        const T2 p = lane_xor2<B>(a, lane);
+        // The FMA forms fetch the partner through the swizzle (LDS pipe),
+        // the add-only rotation form (u1l_bit_rot) through DPP folded into
+        // its adds.
+        const T2 p = lane_xor_swz2<B>(a);
         T2 o;
         if (FORM == FORM_UNIT_DIAG) o = cfma(cp, p, a);
         else if (FORM == FORM_UNIT_IMAG) o = imag_fma(cp.y, p, a);
