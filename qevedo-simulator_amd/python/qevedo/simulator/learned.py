@@ -34,8 +34,8 @@ DEFAULT_MODEL = os.path.join(os.path.dirname(__file__), "..", "..", "..", "share
 
 def training_circuits(qubits: list, seed: int, ising_qubits: list = ()):
     """Random, QFT and kicked-Ising circuits of the given sizes (the families
-    the planner is judged on), plus kicked-Ising circuits at the larger
-    `ising_qubits` sizes, as (name, n, gates)."""
+    the planner's choice is measured on), plus kicked-Ising circuits at the
+    larger `ising_qubits` sizes, as (name, n, gates)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for n in ising_qubits:
