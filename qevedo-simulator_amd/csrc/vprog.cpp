@@ -242,7 +242,13 @@ struct Gen {
         if (qs.size() == 2) {
             const uint32_t b = qs[1];
             pend.d1[b] *= e[2] / e[0];
-            const C64 w = e[3] * e[0] / (e[1] * e[2]);
+            C64 w = e[3] * e[0] / (e[1] * e[2]);
+            // A pair phase within 1e-14 of -1 or 1 is that value: the rounding of
+            // sin and cos of multiples of pi (RZZ(-pi/2), CP(pi)) leaves it an ulp
+            // off, and an exact -1 joins a sign run (a sign flip) instead of a
+            // phase run (a complex product).
+            if (std::abs(w + 1.0) < 1e-14) w = C64(-1, 0);
+            if (std::abs(w - 1.0) < 1e-14) w = C64(1, 0);
             if (w != C64(1, 0)) add_pair(a, b, w);
         }
         return true;
