@@ -472,36 +472,35 @@ __device__ __forceinline__ void phases(T2 (&x)[1 << RB], const QVD_CONST VInstr 
     T2 pr[RB];
 #pragma unroll
     for (int b = 0; b < RB; b++) pr[b] = one;
-    // Pair records (a, b, w in the state's precision), built at upload: one
-    // scalar fetch per pair, none depending on another load.
-    constexpr uint32_t RW = 2 + sizeof(T2) / 4;
-    const uint32_t k = v.k;
-    const QVD_CONST uint32_t *rec = ints + v.inl[0];
-#pragma unroll 4
-    for (uint32_t i = 0; i < k; i++) {
-        const QVD_CONST uint32_t *r = rec + RW * i;
-        const uint32_t a = r[0], b = r[1];
+    // Pair records in the state's precision, built at upload: the pairs of
+    // register bit r as (b, w), inl[1 + r] of them, then the others as
+    // (a, b, w), inl[5] of them. One scalar fetch per pair, none depending
+    // on another load, and no dispatch on the register bit.
+    constexpr uint32_t W = sizeof(T2) / 4;
+    static_assert(RB == 4, "the host writes one pair group per register bit 0-3");
+    auto weight = [&](const QVD_CONST uint32_t *r) {
         T2 w;
         if constexpr (sizeof(T2) == 8) {
-            w.x = __uint_as_float(r[2]);
-            w.y = __uint_as_float(r[3]);
+            w.x = __uint_as_float(r[0]);
+            w.y = __uint_as_float(r[1]);
         } else {
-            w.x = join64(r[2], r[3]);
-            w.y = join64(r[4], r[5]);
+            w.x = join64(r[0], r[1]);
+            w.y = join64(r[2], r[3]);
         }
-        const uint32_t xb = b & 0xffffu;
-        const bool vb = (b & VT_KIND) == VT_THREAD ? ((tid >> xb) & 1) != 0 : ((base >> xb) & 1) != 0;
-        const uint32_t xa = a & 0xffffu;
-        if ((a & VT_KIND) == VT_REG) {
-            const T2 f = sel(vb, w, one);
+        return w;
+    };
+    auto bit = [&](uint32_t e) {
+        const uint32_t xe = e & 0xffffu;
+        return (e & VT_KIND) == VT_THREAD ? ((tid >> xe) & 1) != 0 : ((base >> xe) & 1) != 0;
+    };
+    const QVD_CONST uint32_t *rec = ints + v.inl[0];
 #pragma unroll
-            for (int r = 0; r < RB; r++)
-                if ((uint32_t)r == xa) pr[r] = cm(f, pr[r]);
-        } else {
-            const bool va = (a & VT_KIND) == VT_THREAD ? ((tid >> xa) & 1) != 0 : ((base >> xa) & 1) != 0;
-            t = cm(sel(va && vb, w, one), t);
-        }
+    for (int rb = 0; rb < RB; rb++) {
+        const uint32_t n = v.inl[1 + rb];
+        for (uint32_t i = 0; i < n; i++, rec += 1 + W) pr[rb] = cm(sel(bit(rec[0]), weight(rec + 1), one), pr[rb]);
     }
+    const uint32_t n_other = v.inl[5];
+    for (uint32_t i = 0; i < n_other; i++, rec += 2 + W) t = cm(sel(bit(rec[0]) && bit(rec[1]), weight(rec + 2), one), t);
     // f[j] is live only until its last child; the barriers keep the
     // scheduler from hoisting every product ahead of its use.
     const bool table = v.form == 1;
