@@ -820,13 +820,19 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                 for (int j = 0; j < R; j++) x[j] = cm(cm(ccoef[v.coef + W + j], ft), x[j]);
             } else if constexpr (H == H_XCHG || H == H_LDSOP) {
                 lds_barrier();
+                // The thread part ti and the register part o of a tile index
+                // hold disjoint tile bits, so ti | o = ti + o: one per-thread
+                // base and a uniform offset per register (one add each).
+                {
+                    T2 *const tp = tile + ti;
 #pragma unroll
-                for (int j = 0; j < R; j++) {
-                    uint32_t o = 0;
+                    for (int j = 0; j < R; j++) {
+                        uint32_t o = 0;
 #pragma unroll
-                    for (int b = 0; b < RB; b++)
-                        if ((j >> b) & 1) o |= 1u << rm[b];
-                    tile[ti | o] = x[j];
+                        for (int b = 0; b < RB; b++)
+                            if ((j >> b) & 1) o |= 1u << rm[b];
+                        tp[o] = x[j];
+                    }
                 }
                 lds_barrier();
                 if constexpr (H == H_LDSOP) {
@@ -844,13 +850,14 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
                     for (int b = 0; b < RB; b++) rm[b] = cbyte(v.rmap, b);
                     ti = thread_index(v.lmap, v.wmap);
                 }
+                const T2 *const tq = tile + ti;
 #pragma unroll
                 for (int j = 0; j < R; j++) {
                     uint32_t o = 0;
 #pragma unroll
                     for (int b = 0; b < RB; b++)
                         if ((j >> b) & 1) o |= 1u << rm[b];
-                    x[j] = tile[ti | o];
+                    x[j] = tq[o];
                 }
             }
         };
