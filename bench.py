@@ -4,27 +4,39 @@
 Workload (one step): the 30-qubit depth-20 random circuit (sqrt X / sqrt Y /
 sqrt W on every qubit with the no-repeat rule, then CZ on the brick pattern;
 890 gates) applied in complex64 to |0...0> held in HBM: reset to |0...0>,
-plan and run every stage. Synthetic input,
-generated from seed 42. The engine runs the circuit as register-tile stages
-(2^14 amplitudes per workgroup tile, 32 per thread in VGPRs) plus full-pass
-kernels for any op whose qubits cannot share a stage.
+plan and run every stage. Synthetic input, generated from seed 42. The engine
+runs the circuit as register-tile stages (2^14 amplitudes per workgroup tile,
+16 per thread in VGPRs) plus full-pass kernels for any op whose qubits cannot
+share a stage.
+
+Plans timed on the same circuit and state:
+  heuristic     the engine's heuristic (plan variant 0: region tiles of the
+                circuit's coupling graph)
+  learned       the learned planner over the plans that run the heuristic's
+                arithmetic byte for byte (bit-identical amplitudes): the
+                headline `value`
+  learned_any   the learned planner over every plan variant (other op orders:
+                rounding differs)
+  lookahead     the tile choice of src/blocking.rs's lookahead alone (plan
+                variant 24)
 
 Printed as one JSON line (stdout):
-  value          seconds per circuit (lower is better)
-  roofline       the tile-stage kernel: algorithmic bytes per launch (one read
-                 and one write of the 8 GiB state) over its average duration,
-                 timed with HIP events on the engine's stream in the timed
-                 steps, against the 8 TB/s HBM peak
-  error          complex64 result against the same circuit in complex128 on
-                 the GPU (max-abs, relative L2)
+  value          seconds per circuit of the learned plan (lower is better)
+  roofline       the tile-stage kernel of that plan: algorithmic bytes per
+                 launch (one read and one write of the 8 GiB state) over its
+                 average duration, timed with HIP events on the engine's
+                 stream in the timed steps, against the 8 TB/s HBM peak
+  error          complex64 result against the oracle's FP64 engine on the
+                 host (and against the same circuit in complex128 on the GPU)
   cpu_baseline   the oracle's restatement of the reference CPU path (AVX2
                  blocks, fusion 4, 2^14-block regions, OpenMP) on this host's
-                 cores, same circuit
+                 cores, same circuit, best of 3 after a warm-up
 Progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
 import sys
@@ -52,31 +64,91 @@ def measured_traffic(n: int):
         return json.load(f)["hbm_bytes_per_launch"]
 
 
-def run_batch(count: int) -> dict:
+def host_info() -> dict:
+    """CPU model, logical CPUs (all and the ones this process may use),
+    physical cores and NUMA nodes of this host."""
+    model, cores = "", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and not model:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+            "physical_cores": len(cores) or None,
+            "numa_nodes": len(glob.glob("/sys/devices/system/node/node[0-9]*")) or None}
+
+
+def time_plan(sv, ops, steps: int, warmup: int, **kw) -> dict:
+    """Warm-up runs, then `steps` timed runs (reset + plan + stages), with HIP
+    events around every kernel launch; seconds per circuit and per launch."""
+    for _ in range(warmup):
+        sv.set_zero()
+        sv.apply_ops(ops, **kw)
+    stage_s = other_s = plan_s = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sv.set_zero()
+        st = sv.apply_ops(ops, timing=True, **kw)
+        stage_s += st["stage_kernel_seconds"]
+        other_s += st["other_kernel_seconds"]
+        plan_s += st["plan_seconds"]
+        launches += st["stages"]
+    sec = (time.perf_counter() - t0) / steps
+    return {"seconds": sec, "stage_kernel_seconds": stage_s / steps, "other_kernel_seconds": other_s / steps,
+            "plan_seconds": plan_s / steps, "stage_launch_seconds": stage_s / max(launches, 1), "stats": st}
+
+
+def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
     """Batching workload: `count` hardware-efficient-ansatz circuits (6 layers
     of RY, RZ on every qubit plus a CZ ladder; qubit counts uniform in 12..22,
     angles uniform in [0, 2 pi)) in complex64, <Z_i> and <Z_i Z_i+1> of each:
-    one graph-captured batch against the same circuits run one by one."""
+    one graph-captured batch against the same circuits run one by one on the
+    GPU, and the first `cpu_sample` of them one by one on the oracle's CPU
+    path."""
     import numpy as np
     from qevedo.simulator import Batch, StateVector, library, lower_gates
     from qevedo.simulator import _native
 
     rng = np.random.Generator(np.random.PCG64(2024))
     sizes = [int(x) for x in rng.integers(12, 23, size=count)]
-    circuits = [(n, _native.OpArray(lower_gates(library.hardware_efficient_ansatz(n, 6, 1000 + i))))
-                for i, n in enumerate(sizes)]
+    gate_lists = [library.hardware_efficient_ansatz(n, 6, 1000 + i) for i, n in enumerate(sizes)]
+    circuits = [(n, _native.OpArray(lower_gates(g))) for n, g in zip(sizes, gate_lists)]
     t = time.perf_counter()
     b = Batch(circuits, "single")
     build_s = time.perf_counter() - t
     b.run()
-    reps = 3
+    reps = 5
     t = time.perf_counter()
     for _ in range(reps):
         out = b.run()
     batch_s = (time.perf_counter() - t) / reps
     stats = b.stats
     b.close()
-    # the same circuits one by one: plan, run and expectation values per state
+    # Algorithmic HBM bytes of one replay: every stage of every circuit reads
+    # and writes its state (widened to at least one 14-qubit tile), the reset
+    # writes it and the expectation values read it.
+    passes_bytes = 0
+    for n, ops in circuits:
+        npad = max(n, 14)
+        stages = _native.plan_ops(npad, ops)["stages"]
+        passes_bytes += (2 * stages + 2) * 8 * (1 << npad)
+    achieved = passes_bytes / batch_s / 1e9
+    # the same circuits one by one on the GPU: plan, run and expectation values per state
     t = time.perf_counter()
     worst = 0.0
     for i, (n, ops) in enumerate(circuits):
@@ -86,12 +158,38 @@ def run_batch(count: int) -> dict:
         sv.close()
         worst = max(worst, float(np.max(np.abs(z - out[i][0]))), float(np.max(np.abs(zz - out[i][1]))) if n > 1 else 0.0)
     single_s = time.perf_counter() - t
+    cpu = None
+    if cpu_sample:
+        from oracle import oracle
+        oracle.set_threads(threads)
+        t = time.perf_counter()
+        cworst = 0.0
+        for i in range(min(cpu_sample, count)):
+            n = sizes[i]
+            with oracle.OracleState(n, "single") as o:
+                o.apply_ops(oracle.lower(gate_lists[i]), 4, 14)
+                p = np.abs(o.amplitudes()) ** 2
+            idx = np.arange(1 << n)
+            bits = (idx[:, None] >> np.arange(n)) & 1
+            z = (p[:, None] * (1 - 2 * bits)).sum(axis=0)
+            zz = (p[:, None] * (1 - 2 * (bits[:, :-1] ^ bits[:, 1:]))).sum(axis=0)
+            cworst = max(cworst, float(np.max(np.abs(z - out[i][0]))),
+                         float(np.max(np.abs(zz - out[i][1]))) if n > 1 else 0.0)
+        cpu_s = time.perf_counter() - t
+        k = min(cpu_sample, count)
+        cpu = {"value": cpu_s * count / k, "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"the first {k} of the {count} circuits one by one (oracle apply + <Z>, <ZZ> from the "
+                         f"probabilities), scaled to {count}", "max_abs_diff_vs_batch": cworst}
     log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run ({stats['passes']} stage levels, "
-        f"{stats['stages']} stage descriptors; build {build_s:.2f} s); one by one {single_s:.3f} s; "
-        f"max |diff| {worst:.2e}")
+        f"{stats['stages']} stage descriptors; {achieved:.0f} GB/s algorithmic; build {build_s:.2f} s); "
+        f"one by one on the GPU {single_s:.3f} s; max |diff| {worst:.2e}"
+        + (f"; CPU one by one {cpu['value']:.2f} s (sampled)" if cpu else ""))
     return {"circuits": count, "qubits": "12..22", "seconds": batch_s, "one_by_one_seconds": single_s,
             "speedup": single_s / batch_s, "stage_levels": stats["passes"], "build_seconds": build_s,
-            "max_abs_diff_vs_one_by_one": worst}
+            "max_abs_diff_vs_one_by_one": worst,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_replay": passes_bytes},
+            "cpu_baseline": cpu}
 
 
 def run_precision_form(n: int, depth: int, seed: int) -> dict:
@@ -119,6 +217,7 @@ def run_precision_form(n: int, depth: int, seed: int) -> dict:
     ref.apply_ops(ops, 6, None)
     c128_s = time.perf_counter() - t
     err = sv.compare(ref)
+    norm = sv.norm_sqr()
     ref.close()
     sv.close()
     log(f"precision form {n} q: {st['kernels']} fused ops, complex64 {c64_s:.3f} s (kernels "
@@ -126,14 +225,15 @@ def run_precision_form(n: int, depth: int, seed: int) -> dict:
         f"complex128 {c128_s:.3f} s, max-abs {err['max_abs']:.2e}, rel L2 {err['rel_l2']:.2e}")
     return {"qubits": n, "depth": depth, "fused_ops": st["kernels"], "c64_seconds": c64_s,
             "c64_kernel_seconds": st["other_kernel_seconds"], "c64_valu_kernel_seconds": st_v["other_kernel_seconds"],
-            "c128_seconds": c128_s, "max_abs": err["max_abs"], "rel_l2": err["rel_l2"]}
+            "c128_seconds": c128_s, "max_abs": err["max_abs"], "rel_l2": err["rel_l2"], "c64_norm_sqr": norm}
 
 
 def run_learned_form(n: int, steps: int, seed: int, model_path: str, reps: int = 2) -> dict:
     """Learned form: a kicked-Ising Trotter circuit (RX(theta) on every qubit,
     theta uniform in [0.1, 1.5], then RZZ(-pi/2) on a heavy-hex-like graph),
-    run with the heuristic planner and with the learned one; both final
-    states are kept and compared on the device."""
+    run with the heuristic planner, the learned one (bit-identical plans
+    only), the learned one over every variant and the lookahead planner;
+    the final states are compared on the device."""
     import numpy as np
 
     from qevedo.simulator import StateVector, learned, library, lower_gates
@@ -143,34 +243,38 @@ def run_learned_form(n: int, steps: int, seed: int, model_path: str, reps: int =
     gates = library.kicked_ising(n, library.heavy_hex_edges(n), steps, theta)
     ops = _native.OpArray(lower_gates(gates))
     times = {}
-    states = {}
     net = learned.install(learned.load(model_path))
+    base = StateVector(n, "single")
+    other = StateVector(n, "single")
+    diffs = {}
     try:
-        for name, planner in (("heuristic", _native.PLANNER_HEURISTIC), ("learned", _native.PLANNER_LEARNED)):
-            sv = StateVector(n, "single")
-            st = sv.apply_ops(ops, 4, 0, planner=planner)  # warm-up (and plan choice)
+        for name, kw in (("heuristic", {}), ("learned", {"planner": _native.PLANNER_LEARNED}),
+                         ("learned_any", {"planner": _native.PLANNER_LEARNED_ANY}), ("lookahead", {"variant": 24})):
+            sv = base if name == "heuristic" else other
+            st = sv.apply_ops(ops, 4, 0, **kw)  # warm-up (and plan choice)
             t0 = time.perf_counter()
             for _ in range(reps):
                 sv.set_zero()
-                st = sv.apply_ops(ops, 4, 0, planner=planner)
-            times[name] = (time.perf_counter() - t0) / reps
-            times[name + "_stages"] = st["stages"]
-            times[name + "_variant"] = st["plan_variant"]
-            states[name] = sv
-        diff = states["learned"].compare(states["heuristic"])
+                st = sv.apply_ops(ops, 4, 0, **kw)
+            times[name] = {"seconds": (time.perf_counter() - t0) / reps, "stages": st["stages"],
+                           "variant": st["plan_variant"]}
+            if sv is other:
+                diffs[name] = sv.compare(base)
     finally:
-        for sv in states.values():
-            sv.close()
+        base.close()
+        other.close()
         _native.clear_planner_model()
         net.close()
-    log(f"learned form {n} q kicked Ising ({len(gates)} gates, theta {theta:.3f}): heuristic {times['heuristic']:.3f} s "
-        f"({times['heuristic_stages']} stages), learned {times['learned']:.3f} s (variant {times['learned_variant']}, "
-        f"{times['learned_stages']} stages): {times['heuristic'] / times['learned']:.3f}x; "
-        f"states differ by max-abs {diff['max_abs']:.2e}")
-    return {"qubits": n, "steps": steps, "theta": theta, "gates": len(gates), "heuristic_seconds": times["heuristic"],
-            "learned_seconds": times["learned"], "speedup": times["heuristic"] / times["learned"],
-            "learned_variant": times["learned_variant"], "heuristic_stages": times["heuristic_stages"],
-            "learned_stages": times["learned_stages"], "max_abs_diff": diff["max_abs"], "rel_l2_diff": diff["rel_l2"]}
+    h, l = times["heuristic"]["seconds"], times["learned"]["seconds"]
+    log(f"learned form {n} q kicked Ising ({len(gates)} gates, theta {theta:.3f}): "
+        + ", ".join(f"{k} {v['seconds']:.3f} s (variant {v['variant']}, {v['stages']} stages)" for k, v in times.items())
+        + f"; learned vs heuristic max-abs {diffs['learned']['max_abs']:.1e}")
+    return {"qubits": n, "steps": steps, "theta": theta, "gates": len(gates), "heuristic_seconds": h,
+            "learned_seconds": l, "speedup": h / l, "learned_variant": times["learned"]["variant"],
+            "heuristic_stages": times["heuristic"]["stages"], "learned_stages": times["learned"]["stages"],
+            "max_abs_diff": diffs["learned"]["max_abs"], "rel_l2_diff": diffs["learned"]["rel_l2"],
+            "learned_any": {**times["learned_any"], "max_abs_diff": diffs["learned_any"]["max_abs"]},
+            "lookahead": {**times["lookahead"], "max_abs_diff": diffs["lookahead"]["max_abs"]}}
 
 
 def main() -> int:
@@ -184,13 +288,17 @@ def main() -> int:
     ap.add_argument("--fusion", type=int, default=4)
     ap.add_argument("--tile-bits", type=int, default=0, help="0 = engine default")
     ap.add_argument("--shots", type=int, default=1_000_000)
-    ap.add_argument("--no-check", action="store_true", help="skip the complex128 error check")
+    ap.add_argument("--no-check", action="store_true", help="skip the complex128 and FP64-oracle error checks")
+    ap.add_argument("--no-oracle-check", action="store_true", help="skip the FP64 oracle on the host")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = this process's CPU share (OMP_NUM_THREADS if set, else every usable CPU)")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched hardware-efficient-ansatz workload")
+    ap.add_argument("--batch-cpu-sample", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned planner")
-    ap.add_argument("--precision-form", type=int, default=0, metavar="QUBITS",
-                    help="also run the precision form (fusion to 6-qubit GEMMs on matrix cores) at this size")
+    ap.add_argument("--precision-form", type=int, default=32, metavar="QUBITS",
+                    help="size of the precision form (fusion to 6-qubit GEMMs on matrix cores); 0 = skip")
     ap.add_argument("--batch-circuits", type=int, default=512)
     ap.add_argument("--learned-form", type=int, default=33, metavar="QUBITS",
                     help="kicked-Ising circuit size for the learned-versus-heuristic comparison (0 = skip)")
@@ -208,66 +316,61 @@ def main() -> int:
     from qevedo.simulator import StateVector, library, lower_gates
     from qevedo.simulator import _native
 
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
     n, depth = args.qubits, args.depth
     gates = library.random_circuit(n, depth, args.seed)
     ops = _native.OpArray(lower_gates(gates))
     log(f"{n} qubits, depth {depth}: {len(gates)} gates")
+    kw = {"max_fused_qubits": args.fusion, "tile_bits": args.tile_bits}
 
     sv = StateVector(n, "single")
-    for i in range(args.warmup):
-        sv.set_zero()
-        st = sv.apply_ops(ops, args.fusion, args.tile_bits)
-        log(f"warmup {i}: {st['gate_seconds']:.4f} s, {st['passes']} passes, {st['kernels']} fused ops")
+    plans = {}
+    h = time_plan(sv, ops, args.steps, args.warmup, **kw)
+    plans["heuristic"] = h
+    log(f"heuristic: {h['seconds']:.4f} s/circuit ({h['stats']['stages']} stages, plan {h['plan_seconds'] * 1e3:.1f} ms, "
+        f"stage kernels {h['stage_kernel_seconds']:.4f} s, other kernels {h['other_kernel_seconds']:.4f} s)")
+    heuristic_amps = StateVector(n, "single")
+    heuristic_amps.apply_ops(ops, **kw)
 
-    stage_s = other_s = plan_s = 0.0
-    stage_launches = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sv.set_zero()
-        st = sv.apply_ops(ops, args.fusion, args.tile_bits, timing=True)
-        stage_s += st["stage_kernel_seconds"]
-        other_s += st["other_kernel_seconds"]
-        plan_s += st["plan_seconds"]
-        stage_launches += st["stages"]
-    elapsed = time.perf_counter() - t0
-    sec = elapsed / args.steps
-    log(f"timed {args.steps} steps: {sec:.4f} s/circuit (plan {plan_s / args.steps:.4f} s, "
-        f"stage kernels {stage_s / args.steps:.4f} s, other kernels {other_s / args.steps:.4f} s)")
+    la = time_plan(sv, ops, args.steps, min(args.warmup, 1), variant=24, **kw)
+    plans["lookahead"] = la
+    log(f"lookahead (variant 24): {la['seconds']:.4f} s/circuit ({la['stats']['stages']} stages)")
 
+    # The learned planner (a trained cost model in the tree): over the plans
+    # that reproduce the heuristic's amplitudes (the headline) and over every
+    # variant.
+    model_path = os.path.join(ROOT, "qevedo-simulator_amd", "share", "planner_model.json")
+    have_model = os.path.exists(model_path) and not args.no_learned
+    diffs = {}
+    if have_model:
+        from qevedo.simulator import learned
+        net = learned.install(learned.load(model_path))
+        for name, planner in (("learned", _native.PLANNER_LEARNED), ("learned_any", _native.PLANNER_LEARNED_ANY)):
+            r = time_plan(sv, ops, args.steps, args.warmup, planner=planner, **kw)
+            plans[name] = r
+            diffs[name] = sv.compare(heuristic_amps)
+            log(f"{name}: variant {r['stats']['plan_variant']}, {r['seconds']:.4f} s/circuit "
+                f"({r['stats']['stages']} stages); against the heuristic's state max-abs {diffs[name]['max_abs']:.1e}")
+        _native.clear_planner_model()
+        net.close()
+    heuristic_amps.close()
+    head = plans["learned"] if have_model else h
+    value = head["seconds"]
+    st = head["stats"]
     state_bytes = (1 << n) * 8
     per_launch_bytes = 2 * state_bytes
-    avg_stage = stage_s / max(stage_launches, 1)
-    achieved = per_launch_bytes / avg_stage / 1e9 if stage_launches else 0.0
+    achieved = per_launch_bytes / head["stage_launch_seconds"] / 1e9
 
     result_extra = {"passes": st["passes"], "stages": st["stages"], "fused_ops": st["kernels"], "gates": st["gates"],
-                    "stage_launch_ms": avg_stage * 1e3, "plan_seconds": plan_s / args.steps}
+                    "stage_launch_ms": head["stage_launch_seconds"] * 1e3, "plan_seconds": head["plan_seconds"]}
 
     for _ in range(args.calibration_passes):
         sv.scale(1.0)
 
-    # The learned planner (a trained cost model in the tree): the same circuit
-    # with the plan variant the model predicts fastest.
-    learned_s = None
-    model_path = os.path.join(ROOT, "qevedo-simulator_amd", "share", "planner_model.json")
-    if os.path.exists(model_path) and not args.no_learned:
-        from qevedo.simulator import learned
-        net = learned.install(learned.load(model_path))
-        sv.set_zero()
-        st_l = sv.apply_ops(ops, args.fusion, args.tile_bits, planner=_native.PLANNER_LEARNED)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            sv.set_zero()
-            st_l = sv.apply_ops(ops, args.fusion, args.tile_bits, planner=_native.PLANNER_LEARNED)
-        learned_s = (time.perf_counter() - t0) / args.steps
-        result_extra["learned_variant"] = st_l["plan_variant"]
-        result_extra["learned_plan_seconds"] = st_l["plan_seconds"]
-        log(f"learned planner: variant {st_l['plan_variant']}, {learned_s:.4f} s/circuit "
-            f"(heuristic {sec:.4f} s, plan {st_l['plan_seconds'] * 1e3:.1f} ms)")
-        _native.clear_planner_model()
-        net.close()
-
     # Sampling: 10^6 shots through the two-pass sampler; the first call also
     # sizes the handle's sampling workspace, the second is the steady state.
+    sv.set_zero()
+    sv.apply_ops(ops, **kw)
     if args.shots:
         u = np.random.Generator(np.random.PCG64(7)).random(args.shots)
         first = sv.sample(u)
@@ -283,47 +386,70 @@ def main() -> int:
     if not args.no_check:
         ref = StateVector(n, "double")
         t = time.perf_counter()
-        ref.apply_ops(ops, args.fusion, args.tile_bits)
+        ref.apply_ops(ops, **kw)
         log(f"complex128 run: {time.perf_counter() - t:.3f} s")
-        error = sv.compare(ref)
+        vs_c128 = sv.compare(ref)
+        log(f"complex64 vs GPU complex128: max_abs {vs_c128['max_abs']:.3e} rel_l2 {vs_c128['rel_l2']:.3e}")
+        error = {"vs_gpu_complex128": vs_c128}
+        if not args.no_oracle_check:
+            from oracle import oracle
+            oracle.set_threads(threads)
+            t = time.perf_counter()
+            with oracle.OracleState(n, "double") as o:
+                o.apply_ops(oracle.lower(gates), 4, 14)
+                want = o.amplitudes()
+            fp64 = StateVector(n, "double")
+            fp64.set_amplitudes(want)
+            del want
+            error["max_abs"], error["rel_l2"] = (lambda e: (e["max_abs"], e["rel_l2"]))(sv.compare(fp64))
+            error["gpu_complex128_vs_oracle"] = ref.compare(fp64)
+            error["reference"] = "the oracle's FP64 engine on the host (restated reference CPU path, complex128)"
+            fp64.close()
+            log(f"FP64 oracle on {threads} host threads: {time.perf_counter() - t:.1f} s; complex64 vs oracle "
+                f"max_abs {error['max_abs']:.3e} rel_l2 {error['rel_l2']:.3e}; GPU complex128 vs oracle rel_l2 "
+                f"{error['gpu_complex128_vs_oracle']['rel_l2']:.3e}")
         ref.close()
-        log(f"error vs complex128: max_abs {error['max_abs']:.3e} rel_l2 {error['rel_l2']:.3e}")
 
     # QFT on the same state size, fused to <= 4 qubits.
     qft = _native.OpArray(lower_gates(library.qft(n)))
     sv.set_zero()
-    sv.apply_ops(qft, args.fusion, args.tile_bits)
+    sv.apply_ops(qft, **kw)
     sv.set_zero()
-    st_q = sv.apply_ops(qft, args.fusion, args.tile_bits)
+    st_q = sv.apply_ops(qft, **kw)
     result_extra["qft_seconds"] = st_q["gate_seconds"]
     result_extra["qft_passes"] = st_q["passes"]
     log(f"QFT {n}: {st_q['gate_seconds']:.4f} s, {st_q['passes']} passes")
     sv.close()
 
     if not args.no_batch:
-        result_extra["batch"] = run_batch(args.batch_circuits)
+        result_extra["batch"] = run_batch(args.batch_circuits, 0 if args.no_cpu else args.batch_cpu_sample, threads)
     if args.precision_form:
         result_extra["precision_form"] = run_precision_form(args.precision_form, args.depth, args.seed)
-    if args.learned_form and os.path.exists(model_path) and not args.no_learned:
+    if args.learned_form and have_model:
         result_extra["learned_form"] = run_learned_form(args.learned_form, 20, args.seed, model_path)
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         oracle.set_threads(threads)
         o_ops = oracle.lower(gates)
+        runs = []
         with oracle.OracleState(n, "single") as o:
-            t = time.perf_counter()
-            cst = o.apply_ops(o_ops, 4, 14)
-            cpu_s = time.perf_counter() - t
-        log(f"CPU baseline ({threads} threads): {cpu_s:.3f} s, {cst['passes']} passes")
+            cst = o.apply_ops(o_ops, 4, 14)  # warm-up: pages touched, caches and threads started
+            for _ in range(args.cpu_repeats):
+                t = time.perf_counter()
+                cst = o.apply_ops(o_ops, 4, 14)
+                runs.append(time.perf_counter() - t)
+        cpu_s = min(runs)
+        info = host_info()
+        log(f"CPU baseline ({threads} threads; {info['cpu_model']}, {info['nproc']} logical CPUs, "
+            f"{info['usable_cpus']} usable, {info['physical_cores']} cores, {info['numa_nodes']} NUMA nodes): "
+            f"best {cpu_s:.3f} s of {', '.join(f'{x:.3f}' for x in runs)}, {cst['passes']} passes")
         cpu = {"value": cpu_s, "unit": "s", "cores": threads, "kind": "port",
-               "sample": f"the same {n}-qubit depth-{depth} circuit once, complex64, fusion 4, regions of 2^14 blocks"}
+               "sample": f"the same {n}-qubit depth-{depth} circuit, complex64, fusion 4, regions of 2^14 blocks: "
+                         f"gate time, best of {len(runs)} after a warm-up run",
+               "runs": runs, "host": info, "speedup_vs_gpu": cpu_s / value}
 
-    # The headline is the learned planner's time when a trained model ships
-    # in the tree (the heuristic's time is in "plans"), else the heuristic's.
-    value = learned_s if learned_s is not None else sec
     out = {
         "metric": "30q random-circuit s (learned vs heuristic plan); HBM TB/s; err<=1e-5 vs FP64",
         "value": value,
@@ -339,13 +465,18 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"random circuit, {n} qubits, depth {depth} (sqrt X/Y/W with no repeats + CZ brick), "
                                f"complex64 from |0...0>", "qubits": n, "depth": depth, "gates": len(gates),
-                   "max_fused_qubits": args.fusion, "tile_bits": args.tile_bits or "register tile, 2^14 amplitudes",
-                   "planner": "learned" if learned_s is not None else "heuristic"},
+                   "fusion": "none inside register-tile stages (each stage program merges runs of 1-qubit gates "
+                             f"per qubit); ops outside stages fused to at most {args.fusion} qubits",
+                   "tile_bits": args.tile_bits or "register tile, 2^14 amplitudes",
+                   "planner": "learned (bit-identical to the heuristic)" if have_model else "heuristic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n)},
         "cpu_baseline": cpu,
         "error": error,
-        "plans": {"heuristic": sec, "learned": learned_s},
+        "plans": {k: {"seconds": v["seconds"], "variant": v["stats"]["plan_variant"], "stages": v["stats"]["stages"],
+                      "stage_launch_ms": v["stage_launch_seconds"] * 1e3,
+                      **({"max_abs_diff_vs_heuristic": diffs[k]["max_abs"]} if k in diffs else {})}
+                  for k, v in plans.items()},
     }
     out.update(result_extra)
     print(json.dumps(out), flush=True)

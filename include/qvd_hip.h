@@ -42,7 +42,15 @@ extern "C" {
 
 /* Planner choice for qvd_sv_apply_ops. */
 #define QVD_PLANNER_HEURISTIC 0 /* restated fusion.rs + blocking.rs stage planning */
-#define QVD_PLANNER_LEARNED 1   /* MLP cost model over candidate plans, heuristic fallback */
+/* MLP cost model over the candidate plans that run the same arithmetic as
+ * the heuristic plan (equal plan images, see qvd_plan_equivalent_variants):
+ * the amplitudes are bit-identical to the heuristic's whichever it picks.
+ * Falls back to the heuristic when no model is installed. */
+#define QVD_PLANNER_LEARNED 1
+/* MLP cost model over every plan variant; variants that order ops
+ * differently round differently (complex64 results agree to about 1e-6
+ * relative L2, not bit for bit). Heuristic fallback without a model. */
+#define QVD_PLANNER_LEARNED_ANY 2
 
 typedef struct qvd_sv qvd_sv; /* opaque: HBM amplitudes + a HIP stream */
 
@@ -66,13 +74,14 @@ typedef struct qvd_options {
     uint32_t planner;          /* QVD_PLANNER_* */
     uint32_t flags;            /* QVD_FLAG_* */
     uint32_t variant;          /* heuristic plan variant, 0..QVD_PLAN_VARIANTS-1 (0 = the heuristic);
-                                  QVD_PLANNER_LEARNED chooses it itself */
+                                  the learned planners choose it themselves */
 } qvd_options;
 
 /* Plan variants (blocking.rs's lookahead with a reduced capacity, a greedy
- * choice of the next tile, register slots for the most-used qubits first,
- * tiles chosen among regions of the circuit's coupling graph) and the
- * per-stage features the learned planner scores them by. */
+ * choice of the next tile, register slots in order of need or for the
+ * most-used qubits first, tiles chosen by the lookahead alone or among
+ * regions of the circuit's coupling graph) and the per-stage features the
+ * learned planner scores them by. */
 #define QVD_PLAN_VARIANTS 32
 #define QVD_PLAN_FEATURES 24
 
@@ -191,6 +200,14 @@ int qvd_gate_lower(const char *name, const double *params, uint32_t num_params, 
  * src/blocking.rs:150-302). */
 int qvd_plan_features(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops,
                       const qvd_options *options, float *features, uint64_t capacity, uint64_t *rows);
+/* Which plan variants run exactly the arithmetic of variant options->variant
+ * for these ops: bit v of *mask is set when variant v's plan image (every
+ * register-tile program, stage descriptor, coefficient and full-state op,
+ * byte for byte) equals that of options->variant, so it gives bit-identical
+ * amplitudes. These are the candidates of QVD_PLANNER_LEARNED. New in this
+ * engine (the reference has one planner, src/blocking.rs:230-302). */
+int qvd_plan_equivalent_variants(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops,
+                                 const qvd_options *options, uint32_t *mask);
 /* Kernel time of each register-tile stage of the last qvd_sv_apply_ops run
  * with QVD_FLAG_TIMING, in launch order (seconds, HIP events). */
 int qvd_sv_stage_seconds(const qvd_sv *sv, double *seconds, uint64_t capacity, uint64_t *count);

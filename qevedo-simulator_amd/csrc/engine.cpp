@@ -498,40 +498,106 @@ uint64_t state_cap_bytes() {
     return std::strtoull(e, nullptr, 10);
 }
 
-// Splits register-tile stages whose coefficients would exceed the LDS
-// budget of one stage (each part is a pass).
+// Coefficient bytes the register-tile program of one stage needs in LDS
+// (every stage starts with no pending diagonals: the previous one ends with
+// a scale that applies them).
+size_t stage_coef_bytes(const Stage &st, uint32_t n, int precision, const VGeom &vg) {
+    PendingDiag pend(n);
+    std::vector<C64> co;
+    std::vector<StageOp> lds;
+    std::vector<uint32_t> ints;
+    const VProgram p = build_vprogram(st, n, vg, pend, co, lds, ints, true);
+    return (size_t)p.desc.ccount * (precision == QVD_C64 ? sizeof(float2) : sizeof(double2));
+}
+
+// Splits register-tile stages whose programs would need more coefficients
+// than the LDS block of one stage holds. Each stage is built as the kernel
+// will run it; one that does not fit is cut after the longest run of its
+// ops whose program fits (found by bisection), and the rest is split the
+// same way (each part is a pass over the state).
 std::vector<Step> split_for_coefficients(std::vector<Step> steps, uint32_t n, int precision, const VGeom &vg) {
-    const size_t budget = VCONST_BYTES / (precision == QVD_C64 ? sizeof(float2) : sizeof(double2));
-    const size_t scale = (64u << vg.w_bits) + (1u << vg.r_bits) + 2 * n;
-    auto entries = [](const Op &op) -> size_t {
-        const size_t k = op.targets.size();
-        if (op.diagonal) return size_t(1) << k;
-        return size_t(1) << (2 * k);
-    };
     std::vector<Step> out;
     for (Step &s : steps) {
-        if (!s.is_stage) {
+        if (!s.is_stage || !s.stage.reg_tile || stage_coef_bytes(s.stage, n, precision, vg) <= VCONST_BYTES) {
             out.push_back(std::move(s));
             continue;
         }
-        Step cur;
-        cur.stage.local = s.stage.local;
-        cur.stage.reg_tile = true;
-        cur.stage.role_policy = s.stage.role_policy;
-        size_t used = scale;
-        for (Op &op : s.stage.ops) {
-            const size_t e = entries(op);
-            if (used + e > budget && !cur.stage.ops.empty()) {
-                out.push_back(cur);
-                cur.stage.ops.clear();
-                used = scale;
+        std::vector<Op> rest = std::move(s.stage.ops);
+        while (!rest.empty()) {
+            Step part;
+            part.stage.local = s.stage.local;
+            part.stage.reg_tile = true;
+            part.stage.role_policy = s.stage.role_policy;
+            auto fits = [&](size_t k) {
+                part.stage.ops.assign(rest.begin(), rest.begin() + (ptrdiff_t)k);
+                return stage_coef_bytes(part.stage, n, precision, vg) <= VCONST_BYTES;
+            };
+            size_t lo = 1, hi = rest.size();
+            require(fits(1), "internal: one op's coefficients exceed the LDS budget of a stage");
+            while (lo < hi) {
+                const size_t mid = (lo + hi + 1) / 2;
+                if (fits(mid)) lo = mid;
+                else hi = mid - 1;
             }
-            used += e;
-            cur.stage.ops.push_back(std::move(op));
+            part.stage.ops.assign(rest.begin(), rest.begin() + (ptrdiff_t)lo);
+            rest.erase(rest.begin(), rest.begin() + (ptrdiff_t)lo);
+            out.push_back(std::move(part));
         }
-        if (!cur.stage.ops.empty()) out.push_back(std::move(cur));
     }
     return out;
+}
+
+// Everything the device computes with when it runs a plan, as bytes: each
+// register-tile stage's program (instructions, stage descriptor,
+// coefficients, integer tables and LDS ops) and each full-state op (targets,
+// controls, matrix). The kernels are deterministic, so two plans with equal
+// images give bit-identical amplitudes; plans that apply two gates on
+// different qubits in a different order round differently and have
+// different images.
+std::vector<uint8_t> plan_image(const std::vector<Step> &steps, uint32_t n, int precision) {
+    const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    std::vector<uint8_t> img;
+    auto put = [&](const void *p, size_t bytes) {
+        const uint8_t *b = static_cast<const uint8_t *>(p);
+        img.insert(img.end(), b, b + bytes);
+    };
+    auto put32 = [&](uint32_t x) { put(&x, sizeof x); };
+    PendingDiag pend(n);
+    for (const Step &step : steps) {
+        if (!step.is_stage) {
+            put32(0xffffffffu);
+            put32((uint32_t)step.op.targets.size());
+            put(step.op.targets.data(), step.op.targets.size() * sizeof(uint32_t));
+            put32((uint32_t)step.op.controls.size());
+            for (auto &c : step.op.controls) {
+                put32(c.first);
+                put32(c.second ? 1u : 0u);
+            }
+            put(step.op.m.a.data(), step.op.m.a.size() * sizeof(C64));
+            continue;
+        }
+        require(step.stage.reg_tile, "internal: plan images cover register-tile plans");
+        std::vector<C64> co;
+        std::vector<StageOp> lds;
+        std::vector<uint32_t> ints;
+        const VProgram p = build_vprogram(step.stage, n, vg, pend, co, lds, ints, true);
+        put32((uint32_t)p.instr.size());
+        // the fields up to the last role map (no padding inside that range)
+        for (const VInstr &v : p.instr) put(&v, offsetof(VInstr, rmap) + sizeof v.rmap);
+        const VStageDesc &d = p.desc;
+        put(&d, offsetof(VStageDesc, instr_off));
+        put(&d.num_tiles, offsetof(VStageDesc, cbegin) - offsetof(VStageDesc, num_tiles));
+        put32(d.ccount);
+        put(co.data(), co.size() * sizeof(C64));
+        put32((uint32_t)ints.size());
+        put(ints.data(), ints.size() * sizeof(uint32_t));
+        put32((uint32_t)lds.size());
+        for (const StageOp &o : lds) {
+            put(&o, offsetof(StageOp, gc_mask));  // kind .. lc_val
+            put(&o.gc_mask, sizeof o.gc_mask + sizeof o.gc_val + sizeof o.coef_off);
+        }
+    }
+    return img;
 }
 
 // Features of each register-tile stage of a plan, in launch order: what the
@@ -656,6 +722,30 @@ std::vector<uint32_t> plan_signature(const std::vector<Step> &steps) {
     return sig;
 }
 
+// The variants whose plans have the same image as variant `base` (itself
+// included): the plans that reproduce its amplitudes bit for bit.
+std::vector<uint32_t> equivalent_variants(const std::vector<std::vector<Step>> &plans, uint32_t base, uint32_t n,
+                                          int precision) {
+    std::vector<std::vector<uint8_t>> img(plans.size());
+    std::vector<std::thread> workers;
+    std::vector<std::exception_ptr> errors(plans.size());
+    for (size_t var = 0; var < plans.size(); var++)
+        workers.emplace_back([&, var] {
+            try {
+                img[var] = plan_image(plans[var], n, precision);
+            } catch (...) {
+                errors[var] = std::current_exception();
+            }
+        });
+    for (auto &w : workers) w.join();
+    for (auto &e : errors)
+        if (e) std::rethrow_exception(e);
+    std::vector<uint32_t> out;
+    for (uint32_t var = 0; var < plans.size(); var++)
+        if (img[var] == img[base]) out.push_back(var);
+    return out;
+}
+
 // Validates the ops and plans them for a state of n qubits in `precision`:
 // the heuristic (options.variant picks a variant of it), or, with
 // QVD_PLANNER_LEARNED and a cost model installed, the variant whose stages
@@ -694,12 +784,14 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
     require(opt.variant < PLAN_VARIANTS, "plan variant out of range");
     p.variant = opt.variant;
     if (!p.reg_tile) return plan_heuristic(v, p);
+    require(opt.planner <= QVD_PLANNER_LEARNED_ANY, "unknown planner");
     std::unique_lock<std::mutex> lock(g_model.mu);
-    if (opt.planner != QVD_PLANNER_LEARNED || !g_model.net) {
+    if (opt.planner == QVD_PLANNER_HEURISTIC || !g_model.net) {
         lock.unlock();
         if (chosen) *chosen = p.variant;
         return split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
     }
+    const bool same_amplitudes = opt.planner == QVD_PLANNER_LEARNED;
     // Learned: a choice made for the same ops before is reused.
     const uint64_t key = ops_hash(n, precision, opt, ops, num_ops);
     auto hit = g_model.chosen.find(key);
@@ -709,9 +801,14 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
         return split_for_coefficients(plan_heuristic(v, p), n, precision, vg);
     }
     // Otherwise every variant is planned (in parallel; identical plans are
-    // scored once) and all their stages are scored by the cost model in one
-    // forward pass on the device.
+    // scored once). QVD_PLANNER_LEARNED keeps the variants whose plan image
+    // equals the image of the heuristic plan (options.variant): they run the
+    // same arithmetic, so whichever the model picks, the amplitudes are
+    // bit-identical to the heuristic's. QVD_PLANNER_LEARNED_ANY keeps them
+    // all. The kept plans' stages are scored by the cost model in one forward
+    // pass on the device.
     const uint32_t F = QVD_PLAN_FEATURES;
+    const uint32_t base = p.variant;
     std::vector<std::vector<Step>> plans(PLAN_VARIANTS);
     std::vector<std::vector<float>> feats(PLAN_VARIANTS);
     std::vector<char> dup(PLAN_VARIANTS, 0);
@@ -736,6 +833,11 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
             sigs[var] = plan_signature(plans[var]);
             for (uint32_t u = 0; u < var && !dup[var]; u++)
                 if (!dup[u] && sigs[u] == sigs[var]) dup[var] = 1;
+        }
+        if (same_amplitudes) {
+            std::vector<uint32_t> keep = equivalent_variants(plans, base, n, precision);
+            for (uint32_t var = 0; var < PLAN_VARIANTS; var++)
+                if (std::find(keep.begin(), keep.end(), var) == keep.end()) dup[var] = 1;
         }
         workers.clear();
         for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
@@ -771,17 +873,19 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
             if (!s.is_stage) fixed[var] += 2 * amps * elem / 4.5e12;
     }
     std::vector<float> pred(rows.size() / F);
-    g_model.net->forward(rows.data(), pred.size(), pred.data());
-    uint32_t best = 0;
+    if (!pred.empty()) g_model.net->forward(rows.data(), pred.size(), pred.data());
+    uint32_t best = base;
     double best_t = 0;
+    bool found = false;
     for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
         if (dup[var]) continue;
         double t = fixed[var];
         for (size_t k = first[var]; k < first[var + 1]; k++)
             t += std::exp(g_model.ymean + g_model.yscale * pred[k]) * amps;
-        if (var == 0 || t < best_t) {
+        if (!found || t < best_t) {
             best = var;
             best_t = t;
+            found = true;
         }
     }
     if (g_model.chosen.size() >= 256) g_model.chosen.clear();
@@ -1466,6 +1570,33 @@ int qvd_plan_features(uint32_t num_qubits, int precision, const qvd_op *ops, uin
         *rows = r;
         require(r <= capacity || features == nullptr, "feature buffer too small");
         if (features) std::copy(f.begin(), f.end(), features);
+    });
+}
+
+int qvd_plan_equivalent_variants(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t num_ops,
+                                 const qvd_options *options, uint32_t *mask) {
+    return guard([&] {
+        require(precision == QVD_C64 || precision == QVD_C128, "precision must be QVD_C64 or QVD_C128");
+        require(num_qubits <= 34, "a state vector of more than 34 qubits does not fit in memory");
+        require(mask != nullptr, "null output");
+        qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0, 0};
+        if (options) opt = *options;
+        require(opt.variant < PLAN_VARIANTS, "plan variant out of range");
+        opt.planner = QVD_PLANNER_HEURISTIC;
+        std::vector<std::vector<Step>> plans(PLAN_VARIANTS);
+        for (uint32_t var = 0; var < PLAN_VARIANTS; var++) {
+            qvd_options o = opt;
+            o.variant = var;
+            plans[var] = make_plan(num_qubits, precision, ops, num_ops, o);
+        }
+        *mask = 0;
+        bool reg = true;
+        for (const Step &st : plans[opt.variant]) reg &= !st.is_stage || st.stage.reg_tile;
+        if (!reg) {
+            *mask = 1u << opt.variant;  // LDS-tile plans: only the variant itself
+            return;
+        }
+        for (uint32_t var : equivalent_variants(plans, opt.variant, num_qubits, precision)) *mask |= 1u << var;
     });
 }
 

@@ -149,9 +149,10 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
         return true;
     };
     // Coupling graph of the circuit (qubits that share an op), for the
-    // region candidates of variant bit 4.
+    // region candidates (variant bit 4 clear).
+    const bool regions = (p.variant & 16) == 0;
     std::vector<std::vector<uint32_t>> adj;
-    if (p.variant & 16) {
+    if (regions) {
         adj.assign(n, {});
         for (const Op &op : ops) {
             const std::vector<uint32_t> qs = op.qubits();
@@ -175,7 +176,7 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
             Step s;
             s.stage.local = local;
             s.stage.reg_tile = p.reg_tile;
-            s.stage.role_policy = (p.variant >> 3) & 1;
+            s.stage.role_policy = (p.variant & 8) ? 0 : 1;
             if (p.reg_tile) {
                 // The register-tile program merges 1-qubit runs itself and
                 // keeps gates in unit form; wider fusion would only add
@@ -266,7 +267,7 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
                 }
             }
         }
-        if (p.variant & 16) {
+        if (regions) {
             // Region candidates: for every seed qubit, the breadth-first
             // region of the coupling graph that fills the free capacity (a
             // window along a line, a patch of a lattice). The next tile is the

@@ -50,11 +50,15 @@ struct PlanParams {
     // Plan variant (0 = the heuristic). Bits 0-1: the lookahead's capacity is
     // reduced by that many qubits; bit 2: the next tile is the candidate (four
     // lookahead capacities and the most-targeted qubits) that runs the most
-    // ops; bit 3: register-tile programs give register slots to the most-used
-    // bits first; bit 4: the next tile is the region of the circuit's coupling
-    // graph (one per seed qubit) under which the most ops run next. Every
-    // variant applies the same ops in a dependency-respecting order, so the
-    // amplitudes agree up to rounding.
+    // ops; bit 3: register-tile programs give register slots to bits in order
+    // of need (clear: the most-used bits first); bit 4: the next tile is the
+    // lookahead's alone, as src/blocking.rs:181-226 chooses it (clear: the
+    // lookahead's or, if more pending ops run under it, the region of the
+    // circuit's coupling graph grown from one seed qubit). Every variant
+    // applies the same ops in a dependency-respecting order, so the amplitudes
+    // agree up to rounding; two variants give bit-identical amplitudes only
+    // when they run the same programs, since a different order of two gates
+    // on different qubits rounds differently.
     uint32_t variant = 0;
 };
 

@@ -88,13 +88,23 @@ def _bitstring(key: int, num_clbits: int) -> str:
 
 
 def _count(outcomes: np.ndarray, measures, num_clbits: int) -> dict:
-    """count_outcomes (simulator.rs:187-238): integer keys over the measured
-    (qubit, clbit) pairs, counted, formatted with classical bit 0 on the right."""
-    keys = np.zeros(outcomes.size, dtype=np.uint64)
+    """count_outcomes (simulator.rs:187-238): a key per shot over the measured
+    (qubit, clbit) pairs, counted, formatted with classical bit 0 on the
+    right, in string order. Up to 128 classical bits the key is the OR of the
+    measured bits (the reference's u128 key), kept as two 64-bit words; above
+    128 each pair sets its classical bit, the last one written winning."""
+    outcomes = np.asarray(outcomes, dtype=np.uint64)
+    if num_clbits > 128:
+        bits = np.zeros((outcomes.size, num_clbits), dtype=np.uint8)
+        for q, c in measures:
+            bits[:, c] = (outcomes >> np.uint64(q)) & np.uint64(1)
+        uniq, counts = np.unique(bits, axis=0, return_counts=True)
+        return dict(sorted(("".join("1" if b else "0" for b in row[::-1]), int(n)) for row, n in zip(uniq, counts)))
+    words = np.zeros((outcomes.size, 2), dtype=np.uint64)
     for q, c in measures:
-        keys |= ((outcomes >> np.uint64(q)) & np.uint64(1)) << np.uint64(c)
-    uniq, counts = np.unique(keys, return_counts=True)
-    return {_bitstring(int(k), num_clbits): int(n) for k, n in zip(uniq, counts)}
+        words[:, c >> 6] |= ((outcomes >> np.uint64(q)) & np.uint64(1)) << np.uint64(c & 63)
+    uniq, counts = np.unique(words, axis=0, return_counts=True)
+    return dict(sorted((_bitstring(int(lo) | (int(hi) << 64), num_clbits), int(n)) for (lo, hi), n in zip(uniq, counts)))
 
 
 def _terminal_split(ins) -> int | None:

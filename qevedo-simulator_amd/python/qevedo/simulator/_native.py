@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("QVD_LIB", os.path.join(_PKG_ROOT, "lib", "libqvd_hip.
 
 QVD_OK, QVD_EINVAL, QVD_ENOMEM, QVD_EHIP, QVD_EUNSUPPORTED = 0, -1, -2, -3, -4
 C64, C128 = 0, 1
-PLANNER_HEURISTIC, PLANNER_LEARNED = 0, 1
+PLANNER_HEURISTIC, PLANNER_LEARNED, PLANNER_LEARNED_ANY = 0, 1, 2
 FLAG_TIMING, FLAG_DENSE_VALU = 1, 2
 
 # Every symbol include/qvd_hip.h declares.
@@ -29,7 +29,7 @@ EXPORTS = (
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
     "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
-    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits",
+    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants",
 )
 PLAN_VARIANTS, PLAN_FEATURES = 32, 24
 
@@ -130,6 +130,7 @@ def lib():
         L.qvd_batch_outputs.argtypes = [vp, P(u64), P(u64)]
         f = C.c_float
         L.qvd_plan_features.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(f), u64, P(u64)]
+        L.qvd_plan_equivalent_variants.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(u32)]
         L.qvd_sv_stage_seconds.argtypes = [vp, P(d), u64, P(u64)]
         L.qvd_mlp_create.argtypes = [u32, u32, P(vp)]
         L.qvd_mlp_destroy.argtypes = [vp]
@@ -238,6 +239,18 @@ def plan_features(num_qubits: int, ops, precision: str = "single", variant: int 
     return out[: rows.value]
 
 
+def equivalent_variants(num_qubits: int, ops, precision: str = "single", variant: int = 0,
+                        max_fused_qubits: int = 4) -> list:
+    """The plan variants whose programs equal variant `variant`'s byte for byte
+    (bit-identical amplitudes): the candidates of PLANNER_LEARNED."""
+    arr = ops if isinstance(ops, OpArray) else OpArray(ops)
+    opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0, variant)
+    mask = C.c_uint32()
+    _check(lib().qvd_plan_equivalent_variants(num_qubits, {"single": C64, "double": C128}[precision], arr.arr, arr.n,
+                                              C.byref(opt), C.byref(mask)))
+    return [v for v in range(PLAN_VARIANTS) if (mask.value >> v) & 1]
+
+
 def plan_ops(num_qubits: int, ops, precision: str = "single", max_fused_qubits: int = 4, tile_bits: int | None = 0,
              planner: int = PLANNER_HEURISTIC, variant: int = 0) -> dict:
     """Plan statistics (stages, passes, fused ops) of apply_ops, on the host only."""
@@ -308,9 +321,11 @@ class StateVector:
                   timing: bool = False, variant: int = 0, dense_valu: bool = False) -> dict:
         """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default.
 
-        planner PLANNER_LEARNED scores every plan variant with the installed
-        cost model (heuristic when none is installed); variant picks a
-        heuristic variant directly."""
+        planner PLANNER_LEARNED scores, with the installed cost model, the
+        plan variants that run the heuristic's arithmetic byte for byte
+        (bit-identical amplitudes); PLANNER_LEARNED_ANY scores every variant
+        (other op orders round differently); both are the heuristic when no
+        model is installed. variant picks a heuristic variant directly."""
         arr = ops if isinstance(ops, OpArray) else OpArray(ops)
         flags = (FLAG_TIMING if timing else 0) | (FLAG_DENSE_VALU if dense_valu else 0)
         opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, flags, variant)

@@ -6,7 +6,8 @@ applier, circuits at every fusion width, tiny tiles that force many stages,
 non-tile controls and diagonals, sampling bit-exact against the restated
 two-pass sampler on the same amplitudes and uniforms.
 Tolerances: complex128 1e-12 per op / 1e-11 per circuit, complex64 2e-6 per
-op / 1e-5 per circuit (max-abs), as the reference's tests use.
+op / 1e-5 per circuit (max-abs), as the reference's tests use; circuits also
+meet the relative L2 bounds of 1e-12 (complex128) and 1e-5 (complex64).
 """
 import math
 import sys
@@ -88,7 +89,9 @@ def test_circuits_every_fusion_width(tile_bits):
         for width in range(1, 7):
             with StateVector(n, "double") as s:
                 st = s.apply_ops(ops, width, tile_bits)
-                assert max_distance(s.amplitudes(), expected) < 1e-11, (seed, width)
+                got = s.amplitudes()
+                assert max_distance(got, expected) < 1e-11, (seed, width)
+                assert rel_l2(got, expected) <= 1e-12, (seed, width)
                 assert st["kernels"] <= st["gates"]
             with StateVector(n, "single") as s:
                 s.apply_ops(ops, width, tile_bits)
@@ -108,7 +111,9 @@ def test_tiny_tiles_force_stages():
             for width in (1, 3, 4):
                 with StateVector(n, "double") as s:
                     st = s.apply_ops(ops, width, tile)
-                    assert max_distance(s.amplitudes(), expected) < 1e-11, (seed, tile, width)
+                    got = s.amplitudes()
+                    assert max_distance(got, expected) < 1e-11, (seed, tile, width)
+                    assert rel_l2(got, expected) <= 1e-12, (seed, tile, width)
                     assert st["passes"] > 0
                 with StateVector(n, "single") as s:
                     s.apply_ops(ops, width, tile)
@@ -122,7 +127,9 @@ def test_library_circuits():
         expected = oracle_statevector(n, gates)
         with StateVector(n, "double") as s:
             st = s.apply_ops(lower_gates(gates), 4, 0)
-            assert max_distance(s.amplitudes(), expected) < 1e-12
+            got = s.amplitudes()
+            assert max_distance(got, expected) < 1e-12
+            assert rel_l2(got, expected) <= 1e-12
             assert st["kernels"] < st["gates"]
     # Fusion cuts kernel calls on the fused-op (LDS tile) path; the register
     # tiles apply unfused gates, so this is checked with 13-qubit LDS tiles.
@@ -147,7 +154,9 @@ def test_random_circuit_against_blocked_fp64(n):
     assert rel_l2(got, want) <= 1e-5
     with StateVector(n, "double") as s:
         s.apply_ops(lower_gates(gates), 4, 0)
-        assert max_distance(s.amplitudes(), want) < 1e-11
+        got = s.amplitudes()
+        assert max_distance(got, want) < 1e-11
+        assert rel_l2(got, want) <= 1e-12
 
 
 @pytest.mark.parametrize("precision,tol", [("single", 1e-5), ("double", 1e-11)])
@@ -166,6 +175,7 @@ def test_register_tiles_every_gate_kind(precision, tol):
             st = s.apply_ops(lower_gates(gates))
             got = s.amplitudes()
         assert max_distance(got, expected) < tol, (seed, n, max_distance(got, expected))
+        assert rel_l2(got, expected) <= (1e-5 if precision == "single" else 1e-12), (seed, n)
         assert st["stages"] >= 1
 
 
@@ -195,7 +205,9 @@ def test_register_tiles_structured_circuits(precision):
         expected = oracle_statevector(n, gates)
         with StateVector(n, precision) as s:
             s.apply_ops(lower_gates(gates))
-            assert max_distance(s.amplitudes(), expected) < tol
+            got = s.amplitudes()
+            assert max_distance(got, expected) < tol
+            assert rel_l2(got, expected) <= (1e-5 if precision == "single" else 1e-12)
 
 
 def test_qft_closed_form():

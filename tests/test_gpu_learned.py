@@ -81,15 +81,34 @@ def test_every_variant_gives_the_same_state(name, n, make):
     ref64.close()
 
 
-def test_learned_planner_falls_back_and_chooses_the_predicted_best():
-    n = 20
-    ops = _native.OpArray(lower_gates(library.random_circuit(n, 10, 9)))
-    _native.clear_planner_model()
-    s = StateVector(n, "single")
-    st = s.apply_ops(ops, planner=_native.PLANNER_LEARNED)
-    assert st["plan_variant"] == 0  # no model: the heuristic
-    # A model that predicts log time = sum of the features (all weights
-    # positive) prefers the variant with the fewest, smallest stages.
+IDENTITY_CIRCUITS = CIRCUITS + [
+    ("hea", 20, lambda n: library.hardware_efficient_ansatz(n, 6, 5)),
+]
+
+
+@pytest.mark.parametrize("name,n,make", IDENTITY_CIRCUITS)
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_equivalent_variants_are_bit_identical(name, n, make, precision):
+    """Every candidate of the learned planner (a variant whose plan image
+    equals the heuristic's) gives the heuristic's amplitudes bit for bit."""
+    ops = _native.OpArray(lower_gates(make(n)))
+    eq = _native.equivalent_variants(n, ops, precision)
+    assert 0 in eq
+    base = StateVector(n, precision)
+    base.apply_ops(ops)
+    want = base.amplitudes()
+    base.close()
+    for v in eq:
+        print(f"{name} {n}q {precision} variant {v}", flush=True)
+        s = StateVector(n, precision)
+        s.apply_ops(ops, variant=v)
+        assert np.array_equal(s.amplitudes(), want)
+        s.close()
+
+
+def synthetic_model():
+    """A model that predicts log time = sum of the features (all weights
+    positive): it prefers the plan with the fewest, smallest stages."""
     inputs, hidden = _native.PLAN_FEATURES, 64
     net = _native.Mlp(inputs, hidden)
     W1 = np.full((hidden, inputs), 0.01, np.float32)
@@ -97,12 +116,36 @@ def test_learned_planner_falls_back_and_chooses_the_predicted_best():
                         np.full(hidden, 1.0 / hidden, np.float32), [0.0]]).astype(np.float32)
     net.set_params(p)
     net.install(np.zeros(inputs), np.ones(inputs), -25.0, 1.0)
+    return net
+
+
+def test_learned_planner_falls_back_and_chooses_the_predicted_best():
+    n = 20
+    ops = _native.OpArray(lower_gates(library.random_circuit(n, 10, 9)))
+    _native.clear_planner_model()
+    s = StateVector(n, "single")
+    for planner in (_native.PLANNER_LEARNED, _native.PLANNER_LEARNED_ANY):
+        st = s.apply_ops(ops, planner=planner)
+        assert st["plan_variant"] == 0  # no model: the heuristic
+    s.set_zero()
+    s.apply_ops(ops)
+    heuristic = s.amplitudes()
+    net = synthetic_model()
     scores = []
     for v in range(_native.PLAN_VARIANTS):
         f = _native.plan_features(n, ops, variant=v)
         scores.append(float(np.sum(np.exp(-25.0 + 0.01 * f.sum(axis=1)))))
-    st = s.apply_ops(ops, planner=_native.PLANNER_LEARNED)
+    s.set_zero()
+    st = s.apply_ops(ops, planner=_native.PLANNER_LEARNED_ANY)
     assert st["plan_variant"] == int(np.argmin(scores))
+    # the identity-preserving planner chooses among the equivalent variants
+    # only, and reproduces the heuristic's amplitudes
+    eq = _native.equivalent_variants(n, ops)
+    s.set_zero()
+    st = s.apply_ops(ops, planner=_native.PLANNER_LEARNED)
+    assert st["plan_variant"] in eq
+    assert st["plan_variant"] == min(eq, key=lambda v: (scores[v], v))
+    assert np.array_equal(s.amplitudes(), heuristic)
     _native.clear_planner_model()
     net.close()
     s.close()

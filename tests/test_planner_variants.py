@@ -37,10 +37,59 @@ def test_every_variant_plans_every_op(name, n, gates):
 
 
 def test_capacity_cut_never_plans_fewer_stages():
+    # variants 24..27: the lookahead's tile alone (bit 4), register slots in
+    # order of need (bit 3), capacity cut by 0..3 qubits (bits 0-1)
     n, gates = CIRCUITS[0][1], CIRCUITS[0][2]
     ops = _native.OpArray(lower_gates(gates))
-    stages = [_native.plan_ops(n, ops, variant=v)["stages"] for v in range(4)]
+    stages = [_native.plan_ops(n, ops, variant=v)["stages"] for v in range(24, 28)]
     assert stages == sorted(stages)
+
+
+def test_region_tiles_plan_fewer_stages_than_the_lookahead_alone():
+    # variant 0 (regions of the coupling graph) against variant 24 (the
+    # lookahead's tile only, as src/blocking.rs:181-226 picks local qubits)
+    for name, n, gates in CIRCUITS:
+        ops = _native.OpArray(lower_gates(gates))
+        assert _native.plan_ops(n, ops, variant=0)["stages"] <= _native.plan_ops(n, ops, variant=24)["stages"], name
+
+
+@pytest.mark.parametrize("name,n,gates", CIRCUITS)
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_equivalent_variants_include_the_base_and_agree_both_ways(name, n, gates, precision):
+    ops = _native.OpArray(lower_gates(gates))
+    eq = _native.equivalent_variants(n, ops, precision)
+    assert 0 in eq
+    # equality of plan images is symmetric and transitive: every member sees
+    # the same set
+    for v in eq[1:3]:
+        assert _native.equivalent_variants(n, ops, precision, variant=v) == eq
+    outside = [v for v in range(_native.PLAN_VARIANTS) if v not in eq]
+    if outside:
+        assert 0 not in _native.equivalent_variants(n, ops, precision, variant=outside[0])
+
+
+def test_gate_order_on_different_qubits_changes_float32_rounding():
+    """Why the candidates of the learned planner must run the same programs:
+    two 1-qubit gates on different qubits commute exactly, but applied in the
+    other order their float32 arithmetic rounds differently, so a plan that
+    reorders them cannot reproduce another plan's amplitudes bit for bit."""
+    r = np.random.Generator(np.random.PCG64(11))
+    psi = (r.standard_normal((2, 2, 2)) + 1j * r.standard_normal((2, 2, 2))).astype(np.complex64)
+
+    def rot(theta, axis):
+        c, s = np.cos(theta / 2), np.sin(theta / 2)
+        m = np.array([[c, -1j * s], [-1j * s, c]]) if axis == "x" else np.array([[c, -s], [s, c]])
+        return m.astype(np.complex64)
+
+    def apply(state, m, axis):
+        # float32 complex arithmetic, one gate at a time
+        return np.moveaxis(np.tensordot(m, np.moveaxis(state, axis, 0), axes=1), 0, axis).astype(np.complex64)
+
+    a, b = rot(0.7, "x"), rot(1.9, "y")
+    ab = apply(apply(psi, a, 2), b, 1)
+    ba = apply(apply(psi, b, 1), a, 2)
+    assert np.allclose(ab, ba, atol=1e-6)
+    assert not np.array_equal(ab, ba)
 
 
 def test_variant_out_of_range_is_an_error():

@@ -122,3 +122,22 @@ def test_mid_circuit_measurement_reset_and_feedback():
     c = Circuit(1).h(0).measure(0, 0).feedback("X", 0, 0).measure(0, 1)
     result = run(c, shots=300, seed=4)
     assert set(result.counts) <= {"00", "01"} and len(result.counts) == 2
+
+
+@pytest.mark.gpu
+def test_sampling_after_relabeling_keys():
+    """tests/simulator.rs:195-214: x(9) x(7) h(0) cx(0, 8), measured: the
+    samples are reported in logical order, so the only keys are exactly
+    1010000000 and 1110000001; also with tiles of 1-3 qubits, which force a
+    stage or full-state pass per op."""
+    from qevedo.simulator import StateVector, _count, lower_gates
+
+    c = Circuit(10).x(9).x(7).h(0).cx(0, 8).measure_all()
+    assert sorted(run(c, 1000, seed=1).counts) == ["1010000000", "1110000001"]
+    gates = [("x", (9,), ()), ("x", (7,), ()), ("h", (0,), ()), ("cx", (0, 8), ())]
+    u = np.random.Generator(np.random.PCG64(1)).random(1000)
+    for tile in (1, 2, 3):
+        with StateVector(10, "single") as s:
+            s.apply_ops(lower_gates(gates), 4, tile)
+            keys = sorted(_count(s.sample(u), [(q, q) for q in range(10)], 10))
+        assert keys == ["1010000000", "1110000001"], tile
