@@ -296,15 +296,17 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
                 std::vector<uint32_t> cand = tile_of(region);
                 if (std::find(cands.begin(), cands.end(), cand) == cands.end()) cands.push_back(std::move(cand));
             }
-            size_t best = 0;
-            bool found = false;
+            // A region replaces the lookahead's tile only when more pending
+            // ops run under it; one under which nothing runs (an op wider
+            // than a stage is next) never does, so such an op still gets its
+            // own full-state pass below.
+            size_t best = next == local ? 0 : runnable(next);
             for (const auto &cnd : cands) {
                 if (cnd == local) continue;
                 const size_t k = runnable(cnd);
-                if (!found || k > best) {
+                if (k > best) {
                     best = k;
                     next = cnd;
-                    found = true;
                 }
             }
         }

@@ -96,3 +96,16 @@ def test_variant_out_of_range_is_an_error():
     ops = _native.OpArray(lower_gates(library.qft(16)))
     with pytest.raises(NativeError, match="variant"):
         _native.plan_ops(16, ops, variant=_native.PLAN_VARIANTS)
+
+
+@pytest.mark.parametrize("variant", [0, 4, 16, 24])
+def test_op_wider_than_a_stage_gets_its_own_pass(variant):
+    """An op on more targets than a stage applies (5 here) runs as a
+    full-state pass between stages under every tile-choice rule."""
+    import numpy as np
+    from helpers import random_unitary, rng
+
+    m = random_unitary(rng(3), 5)
+    ops = [([0, 3, 6, 9, 12], [], m), ([1], [], np.eye(2)), ([0, 4, 8, 12, 15], [], m)]
+    st = _native.plan_ops(16, ops, variant=variant)
+    assert st["gates"] == 3 and st["passes"] >= 2 and st["passes"] - st["stages"] == 2
