@@ -169,10 +169,15 @@ def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
             with oracle.OracleState(n, "single") as o:
                 o.apply_ops(oracle.lower(gate_lists[i]), 4, 14)
                 p = np.abs(o.amplitudes()) ** 2
-            idx = np.arange(1 << n)
-            bits = (idx[:, None] >> np.arange(n)) & 1
-            z = (p[:, None] * (1 - 2 * bits)).sum(axis=0)
-            zz = (p[:, None] * (1 - 2 * (bits[:, :-1] ^ bits[:, 1:]))).sum(axis=0)
+            # marginals of neighbouring qubit pairs: p summed over every
+            # other qubit, as a (b_{q+1}, b_q) table
+            z, zz = np.zeros(n), np.zeros(max(n - 1, 0))
+            for q in range(n - 1):
+                m = p.reshape(1 << (n - q - 2), 2, 2, 1 << q).sum(axis=(0, 3))
+                zz[q] = m[0, 0] + m[1, 1] - m[0, 1] - m[1, 0]
+                z[q] = m[0, 0] + m[1, 0] - m[0, 1] - m[1, 1]
+                if q == n - 2:
+                    z[q + 1] = m[0, 0] + m[0, 1] - m[1, 0] - m[1, 1]
             cworst = max(cworst, float(np.max(np.abs(z - out[i][0]))),
                          float(np.max(np.abs(zz - out[i][1]))) if n > 1 else 0.0)
         cpu_s = time.perf_counter() - t

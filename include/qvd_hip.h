@@ -155,8 +155,17 @@ int qvd_sv_norm_sqr(const qvd_sv *sv, double *out);
 int qvd_sv_scale(qvd_sv *sv, double factor);
 /* probability_one (src/measure.rs:43-65), FP64. */
 int qvd_sv_probability_one(const qvd_sv *sv, uint32_t qubit, double *out);
-/* probabilities (src/measure.rs:25-40): 2^n FP64 values in logical order. */
+/* probabilities (src/measure.rs:25-40): 2^n FP64 values in logical order,
+ * computed on the device. */
 int qvd_sv_probabilities(const qvd_sv *sv, double *out, uint64_t len);
+/* <P_t> for Pauli strings P_t, FP64 on the device: the estimator's
+ * statevector path (python/python/qevedo/simulator/qiskit.py:584-589, where
+ * Qiskit's Statevector.expectation_value evaluates a SparsePauliOp term by
+ * term). String t has X on the qubits of x_masks[t] only, Z on those of
+ * z_masks[t] only and Y on those of both (bit q = qubit q). Strings with the
+ * same X part share one pass over the state (up to 16 per pass). */
+int qvd_sv_expect_pauli(const qvd_sv *sv, const uint64_t *x_masks, const uint64_t *z_masks, uint64_t num_terms,
+                        double *out);
 /* <Z_q> for every qubit (n values) and <Z_q Z_{q+1}> (n-1 values), FP64.
  * New in this engine; <Z_q> = 1 - 2 probability_one(q). */
 int qvd_sv_expect_z(const qvd_sv *sv, double *z, double *zz);

@@ -1054,6 +1054,57 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     QVD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 }
 
+// The sampler's buffers carved from one workspace (256-byte aligned parts);
+// returns the bytes needed, and fills `w`, the uniforms' and the outputs'
+// device addresses when `base` is not null.
+size_t carve_sample(const qvd_sv *sv, uint64_t shots, char *base, SampleScratch &w, double **du, uint64_t **dout) {
+    // Chunks of the reference's CHUNK_BLOCKS blocks (src/measure.rs:12-13):
+    // 4096 x 8 amplitudes in complex64, 4096 x 4 in complex128.
+    const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
+    const uint64_t nc = (sv->len + chunk - 1) / chunk;
+    w.sort_temp_bytes = sort_temp_bytes(shots);
+    require(w.sort_temp_bytes > 0, "radix sort temporary size query failed");
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t o_sums = carve(nc * sizeof(double)), o_prefix = carve((nc + 1) * sizeof(double));
+    const size_t o_bounds = carve((nc + 1) * sizeof(uint64_t)), o_tu = carve(shots * sizeof(double));
+    const size_t o_ts = carve(shots * sizeof(double)), o_du = carve(shots * sizeof(double));
+    const size_t o_out = carve(shots * sizeof(uint64_t)), o_temp = carve(w.sort_temp_bytes);
+    if (base) {
+        w.sums = reinterpret_cast<double *>(base + o_sums);
+        w.prefix = reinterpret_cast<double *>(base + o_prefix);
+        w.bounds = reinterpret_cast<uint64_t *>(base + o_bounds);
+        w.targets_unsorted = reinterpret_cast<double *>(base + o_tu);
+        w.targets = reinterpret_cast<double *>(base + o_ts);
+        w.sort_temp = base + o_temp;
+        *du = reinterpret_cast<double *>(base + o_du);
+        *dout = reinterpret_cast<uint64_t *>(base + o_out);
+    }
+    return off;
+}
+
+// Two-pass sampling with the buffers carved from `base` (at least
+// carve_sample bytes of device memory): copies the uniforms in, runs the
+// sampler on the handle's stream, copies the outcomes out and waits.
+void sample_with(qvd_sv *sv, const double *uniforms, uint64_t shots, uint64_t *out, char *base) {
+    const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
+    SampleScratch w;
+    double *du = nullptr;
+    uint64_t *dout = nullptr;
+    carve_sample(sv, shots, base, w, &du, &dout);
+    QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
+    with_type(sv, [&](auto *s) {
+        QVD_HIP(launch_sample(s, sv->len, chunk, du, shots, w, dout, sv->stream));
+        return 0;
+    });
+    QVD_HIP(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
+    sv->sync();
+}
+
 }  // namespace
 }  // namespace qvd
 
@@ -1322,15 +1373,77 @@ int qvd_sv_probability_one(const qvd_sv *sv_c, uint32_t qubit, double *out) {
     });
 }
 
-int qvd_sv_probabilities(const qvd_sv *sv, double *out, uint64_t len) {
+int qvd_sv_probabilities(const qvd_sv *sv_c, double *out, uint64_t len) {
     return guard([&] {
-        check_handle(sv);
+        check_handle(sv_c);
         require(out != nullptr, "null output");
-        require(len == sv->len, "length does not match the state");
-        std::vector<double> a(2 * len);
-        int rc = qvd_sv_get_amplitudes(sv, a.data(), len);
-        if (rc != QVD_OK) fail(rc, qvd::g_last_error);
-        for (uint64_t i = 0; i < len; i++) out[i] = a[2 * i] * a[2 * i] + a[2 * i + 1] * a[2 * i + 1];
+        require(len == sv_c->len, "length does not match the state");
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        sv->bind();
+        // |a|^2 in FP64 on the device, copied out in pieces of at most 2^24
+        // values (128 MiB of staging).
+        const uint64_t piece = std::min<uint64_t>(len, uint64_t(1) << 24);
+        DBuf buf(piece * sizeof(double));
+        for (uint64_t off = 0; off < len; off += piece) {
+            const uint64_t cnt = std::min(piece, len - off);
+            with_type(sv, [&](auto *s) {
+                launch_probabilities(s + off, cnt, buf.as<double>(), sv->stream);
+                return 0;
+            });
+            QVD_HIP(hipGetLastError());
+            QVD_HIP(hipMemcpyAsync(out + off, buf.p, cnt * sizeof(double), hipMemcpyDeviceToHost, sv->stream));
+            sv->sync();
+        }
+    });
+}
+
+int qvd_sv_expect_pauli(const qvd_sv *sv_c, const uint64_t *x_masks, const uint64_t *z_masks, uint64_t num_terms,
+                        double *out) {
+    return guard([&] {
+        check_handle(sv_c);
+        if (num_terms == 0) return;
+        require(x_masks != nullptr && z_masks != nullptr && out != nullptr, "null masks or output");
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        const uint64_t all = sv->n == 64 ? ~uint64_t(0) : (uint64_t(1) << sv->n) - 1;
+        for (uint64_t t = 0; t < num_terms; t++)
+            require(((x_masks[t] | z_masks[t]) & ~all) == 0, "Pauli string acts on a qubit out of range");
+        sv->bind();
+        // Terms grouped by their X part: each group of up to PAULI_TERMS
+        // terms is one pass over the state.
+        std::vector<uint64_t> order(num_terms);
+        for (uint64_t t = 0; t < num_terms; t++) order[t] = t;
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return x_masks[a] < x_masks[b]; });
+        const unsigned blocks = 1024;
+        const uint32_t W = 2 * PAULI_TERMS;
+        double *buf = sv->scratch_doubles((size_t)blocks * W + W + PAULI_TERMS);
+        double *partial = buf, *sums = buf + (size_t)blocks * W;
+        uint64_t *dz = reinterpret_cast<uint64_t *>(sums + W);
+        std::vector<uint64_t> zs;
+        std::vector<double> h(W);
+        for (uint64_t g = 0; g < num_terms;) {
+            const uint64_t x = x_masks[order[g]];
+            zs.clear();
+            uint64_t e = g;
+            while (e < num_terms && x_masks[order[e]] == x && zs.size() < PAULI_TERMS) zs.push_back(z_masks[order[e++]]);
+            const uint32_t k = (uint32_t)zs.size();
+            QVD_HIP(hipMemcpyAsync(dz, zs.data(), k * sizeof(uint64_t), hipMemcpyHostToDevice, sv->stream));
+            with_type(sv, [&](auto *s) {
+                launch_expect_pauli(s, sv->len, x, dz, k, partial, blocks, sums, sv->stream);
+                return 0;
+            });
+            QVD_HIP(hipGetLastError());
+            QVD_HIP(hipMemcpyAsync(h.data(), sums, 2 * k * sizeof(double), hipMemcpyDeviceToHost, sv->stream));
+            sv->sync();
+            for (uint32_t t = 0; t < k; t++) {
+                // P = i^popcount(x & z) X^x Z^z; its expectation is real for
+                // a Hermitian string, so only the real part is kept.
+                const uint64_t idx = order[g + t];
+                const int ny = __builtin_popcountll(x & z_masks[idx]) & 3;
+                const double re = h[2 * t], im = h[2 * t + 1];
+                out[idx] = ny == 0 ? re : ny == 1 ? -im : ny == 2 ? -re : im;
+            }
+            g = e;
+        }
     });
 }
 
@@ -1411,42 +1524,12 @@ int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, ui
         require(shots < (uint64_t(1) << 31), "at most 2^31 - 1 shots per call");
         sv->bind();
         const double t0 = now();
-        // Chunks of the reference's CHUNK_BLOCKS blocks (src/measure.rs:12-13):
-        // 4096 x 8 amplitudes in complex64, 4096 x 4 in complex128.
-        const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
-        const uint64_t nc = (sv->len + chunk - 1) / chunk;
+        // One workspace per handle, grown to the largest call and kept: no
+        // allocation on repeated calls.
         SampleScratch w;
-        w.sort_temp_bytes = sort_temp_bytes(shots);
-        require(w.sort_temp_bytes > 0, "radix sort temporary size query failed");
-        // One workspace per handle, carved into the sampler's buffers (256-byte
-        // aligned) and kept for the next call: no allocation on repeated calls.
-        size_t off = 0;
-        auto carve = [&](size_t bytes) {
-            const size_t at = off;
-            off += (bytes + 255) & ~size_t(255);
-            return at;
-        };
-        const size_t o_sums = carve(nc * sizeof(double)), o_prefix = carve((nc + 1) * sizeof(double));
-        const size_t o_bounds = carve((nc + 1) * sizeof(uint64_t)), o_tu = carve(shots * sizeof(double));
-        const size_t o_ts = carve(shots * sizeof(double)), o_du = carve(shots * sizeof(double));
-        const size_t o_out = carve(shots * sizeof(uint64_t)), o_temp = carve(w.sort_temp_bytes);
-        if (sv->sample_ws.bytes < off) sv->sample_ws.alloc(off);
-        char *base = sv->sample_ws.as<char>();
-        w.sums = reinterpret_cast<double *>(base + o_sums);
-        w.prefix = reinterpret_cast<double *>(base + o_prefix);
-        w.bounds = reinterpret_cast<uint64_t *>(base + o_bounds);
-        w.targets_unsorted = reinterpret_cast<double *>(base + o_tu);
-        w.targets = reinterpret_cast<double *>(base + o_ts);
-        w.sort_temp = base + o_temp;
-        double *du = reinterpret_cast<double *>(base + o_du);
-        uint64_t *dout = reinterpret_cast<uint64_t *>(base + o_out);
-        QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
-        with_type(sv, [&](auto *s) {
-            QVD_HIP(launch_sample(s, sv->len, chunk, du, shots, w, dout, sv->stream));
-            return 0;
-        });
-        QVD_HIP(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
-        sv->sync();
+        const size_t need = carve_sample(sv, shots, nullptr, w, nullptr, nullptr);
+        if (sv->sample_ws.bytes < need) sv->sample_ws.alloc(need);
+        sample_with(sv, uniforms, shots, out, sv->sample_ws.as<char>());
         sv->last_sample_seconds = now() - t0;
     });
 }

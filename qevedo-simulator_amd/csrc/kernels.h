@@ -25,7 +25,15 @@ struct SampleScratch {
     size_t sort_temp_bytes = 0;
 };
 
+// Pauli strings per pass of the expectation kernel (one X part, up to this
+// many Z parts).
+constexpr uint32_t PAULI_TERMS = 16;
+
 template <typename T2> void launch_init_zero(T2 *s, uint64_t len, hipStream_t st);
+template <typename T2> void launch_probabilities(const T2 *s, uint64_t len, double *out, hipStream_t st);
+template <typename T2>
+void launch_expect_pauli(const T2 *s, uint64_t len, uint64_t x, const uint64_t *zs, uint32_t nterms, double *partial,
+                         unsigned blocks, double *out, hipStream_t st);
 template <typename T2> void launch_widen(const T2 *s, double2 *out, uint64_t count, hipStream_t st);
 template <typename T2> void launch_narrow(const double2 *in, T2 *s, uint64_t count, hipStream_t st);
 template <typename T2> void launch_scale(T2 *s, uint64_t len, double factor, hipStream_t st);

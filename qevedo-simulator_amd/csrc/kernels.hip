@@ -316,6 +316,64 @@ __global__ __launch_bounds__(RED_THREADS) void k_masked_norm(const T2 *__restric
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
+// |a_i|^2 in FP64, in index order.
+template <typename T2>
+__global__ void k_probabilities(const T2 *__restrict__ s, uint64_t len, double *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) out[i] = prob(s[i]);
+}
+
+// Pauli strings sharing one X part x (qubits with X or Y) and differing in
+// their Z parts z[t] (qubits with Z or Y): per block, the sums over j of
+// (-1)^popcount((j ^ x) & z[t]) conj(a_j) a_(j ^ x), real and imaginary
+// parts in FP64, written to partial[block * 2 * nterms + 2 t (+1)]. With the
+// phase i^popcount(x & z[t]) applied afterwards this is <P_t>.
+template <typename T2>
+__global__ __launch_bounds__(RED_THREADS) void k_expect_pauli(const T2 *__restrict__ s, uint64_t len, uint64_t x,
+                                                              const uint64_t *__restrict__ zs, uint32_t nterms,
+                                                              double *__restrict__ partial) {
+    __shared__ double scratch[RED_THREADS / 64];
+    double re[PAULI_TERMS], im[PAULI_TERMS];
+    uint64_t z[PAULI_TERMS];
+#pragma unroll
+    for (int t = 0; t < PAULI_TERMS; t++) {
+        re[t] = im[t] = 0;
+        z[t] = t < (int)nterms ? zs[t] : 0;
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += stride) {
+        const T2 a = s[j], b = s[j ^ x];
+        const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+        const double vr = ar * br + ai * bi, vi = ar * bi - ai * br;  // conj(a) b
+        const uint64_t k = j ^ x;
+#pragma unroll
+        for (int t = 0; t < PAULI_TERMS; t++) {  // every slot: unused ones have z = 0 and are not written
+            const bool neg = __popcll(k & z[t]) & 1;
+            re[t] += neg ? -vr : vr;
+            im[t] += neg ? -vi : vi;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < PAULI_TERMS; t++) {
+        const double r = block_sum(re[t], scratch);
+        const double i = block_sum(im[t], scratch);
+        if (threadIdx.x == 0 && t < (int)nterms) {
+            partial[(uint64_t)blockIdx.x * 2 * nterms + 2 * t] = r;
+            partial[(uint64_t)blockIdx.x * 2 * nterms + 2 * t + 1] = i;
+        }
+    }
+}
+
+// out[v] = sum over blocks b, in block order, of partial[b * width + v].
+__global__ void k_sum_columns(const double *__restrict__ partial, uint32_t blocks, uint32_t width,
+                              double *__restrict__ out) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= width) return;
+    double r = 0;
+    for (uint32_t b = 0; b < blocks; b++) r += partial[(uint64_t)b * width + v];
+    out[v] = r;
+}
+
 // Sequential sum of partial[0..m) into out[0].
 __global__ void k_sum_partials(const double *__restrict__ partial, uint32_t m, double *__restrict__ out) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -737,6 +795,18 @@ void launch_masked_norm(const T2 *s, uint64_t len, uint64_t mask, uint64_t val, 
     k_sum_partials<<<1, 64, 0, st>>>(partial, blocks, out);
 }
 
+template <typename T2>
+void launch_probabilities(const T2 *s, uint64_t len, double *out, hipStream_t st) {
+    k_probabilities<T2><<<grid_for(len, 256), 256, 0, st>>>(s, len, out);
+}
+
+template <typename T2>
+void launch_expect_pauli(const T2 *s, uint64_t len, uint64_t x, const uint64_t *zs, uint32_t nterms, double *partial,
+                         unsigned blocks, double *out, hipStream_t st) {
+    k_expect_pauli<T2><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
+    k_sum_columns<<<1, 64, 0, st>>>(partial, blocks, 2 * nterms, out);
+}
+
 ExpDesc expect_desc(uint32_t state_qubits, uint32_t n) {
     ExpDesc d{};
     d.B = expect_segment_bits(state_qubits);
@@ -824,6 +894,9 @@ hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double
     template void launch_masked_norm<T2>(const T2 *, uint64_t, uint64_t, uint64_t, double *, unsigned, double *, \
                                          hipStream_t);                                                          \
     template void launch_expect_z<T2>(const T2 *, uint32_t, double *, double *, double *, hipStream_t);         \
+    template void launch_probabilities<T2>(const T2 *, uint64_t, double *, hipStream_t);                        \
+    template void launch_expect_pauli<T2>(const T2 *, uint64_t, uint64_t, const uint64_t *, uint32_t, double *, \
+                                          unsigned, double *, hipStream_t);                                     \
     template void launch_expect_batch<T2>(const T2 *, const ExpDesc *, uint32_t, const uint64_t *, uint64_t,    \
                                           const uint32_t *, uint32_t, double *, double *, double *, hipStream_t); \
     template void launch_batch_init<T2>(T2 *, uint64_t, const uint64_t *, uint32_t, hipStream_t);              \

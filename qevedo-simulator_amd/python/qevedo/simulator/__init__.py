@@ -27,8 +27,8 @@ from .circuit import Circuit
 
 __version__ = "0.1.18+mi355x"
 
-__all__ = ["Batch", "Circuit", "NativeError", "RunResult", "StateVector", "choose_backend", "device_info", "gate_lower",
-           "gate_matrix", "library", "lower_gates", "plan_ops", "run", "statevector"]
+__all__ = ["Batch", "Circuit", "NativeError", "RunResult", "StateVector", "choose_backend", "device_info", "expectation",
+           "gate_lower", "gate_matrix", "library", "lower_gates", "plan_ops", "run", "statevector"]
 
 BACKENDS = ("auto", "statevector")
 _STATE_ALIASES = ("statevector", "state_vector")
@@ -202,3 +202,20 @@ def statevector(circuit, *, threads=None) -> np.ndarray:
         if ops:
             sv.apply_ops(ops)
         return sv.amplitudes()
+
+
+def expectation(circuit, observable, *, precision: str = "double") -> float:
+    """<psi|O|psi> for the final state of a unitary circuit and an observable
+    given as {Pauli label: coefficient} (Qiskit labels: the last character
+    is qubit 0): the estimator's statevector path
+    (python/python/qevedo/simulator/qiskit.py:584-589), with the state and
+    every Pauli term evaluated in HBM (FP64 sums)."""
+    c = _circuit(circuit)
+    for kind, *_ in c._ins:
+        if kind not in ("gate", "unitary", "barrier"):
+            raise ValueError(f"expectation() needs a unitary circuit, found {kind}")
+    with StateVector(c.num_qubits, precision) as sv:
+        ops = c.ops()
+        if ops:
+            sv.apply_ops(ops)
+        return sv.expectation_value(observable)

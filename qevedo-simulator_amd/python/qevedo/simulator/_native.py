@@ -29,7 +29,7 @@ EXPORTS = (
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
     "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
-    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants",
+    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_expect_pauli",
 )
 PLAN_VARIANTS, PLAN_FEATURES = 32, 24
 
@@ -116,6 +116,7 @@ def lib():
         L.qvd_sv_probability_one.argtypes = [vp, u32, P(d)]
         L.qvd_sv_probabilities.argtypes = [vp, P(d), u64]
         L.qvd_sv_expect_z.argtypes = [vp, P(d), P(d)]
+        L.qvd_sv_expect_pauli.argtypes = [vp, P(u64), P(u64), u64, P(d)]
         L.qvd_sv_measure.argtypes = [vp, u32, d, P(C.c_uint8)]
         L.qvd_sv_reset.argtypes = [vp, u32, d]
         L.qvd_sv_sample.argtypes = [vp, P(d), u64, P(u64)]
@@ -261,6 +262,33 @@ def plan_ops(num_qubits: int, ops, precision: str = "single", max_fused_qubits: 
     return st.as_dict()
 
 
+def pauli_masks(label: str) -> tuple:
+    """(x_mask, z_mask) of a Pauli label in Qiskit's order (the last
+    character acts on qubit 0)."""
+    x = z = 0
+    for q, ch in enumerate(reversed(label.upper())):
+        if ch in "XY":
+            x |= 1 << q
+        if ch in "ZY":
+            z |= 1 << q
+        if ch not in "IXYZ":
+            raise ValueError(f"Pauli label {label!r}: {ch!r} is not one of I, X, Y, Z")
+    return x, z
+
+
+def pauli_terms(observable, num_qubits: int) -> tuple:
+    """Labels and coefficients of an observable given as a {label:
+    coefficient} mapping or a sequence of (label, coefficient) pairs."""
+    items = list(observable.items()) if hasattr(observable, "items") else list(observable)
+    labels, coeffs = [], []
+    for label, c in items:
+        if len(label) != num_qubits:
+            raise ValueError(f"Pauli label {label!r} has {len(label)} qubits, the state {num_qubits}")
+        labels.append(label)
+        coeffs.append(complex(c))
+    return labels, coeffs
+
+
 class StateVector:
     """A state of `num_qubits` qubits in HBM (StateVector, src/state.rs).
 
@@ -365,6 +393,27 @@ class StateVector:
         zz = np.zeros(max(n - 1, 1), dtype=np.float64)
         _check(lib().qvd_sv_expect_z(self.h, _p(z, C.c_double), _p(zz, C.c_double)))
         return z[:n], zz[: max(n - 1, 0)]
+
+    def expect_pauli(self, x_masks, z_masks) -> np.ndarray:
+        """<P_t> in FP64 for Pauli strings given as bit masks (bit q = qubit
+        q): X on x-only qubits, Z on z-only qubits, Y on qubits in both."""
+        x = np.ascontiguousarray(x_masks, dtype=np.uint64)
+        z = np.ascontiguousarray(z_masks, dtype=np.uint64)
+        if x.shape != z.shape:
+            raise ValueError("x_masks and z_masks must have the same length")
+        out = np.zeros(max(x.size, 1), dtype=np.float64)
+        _check(lib().qvd_sv_expect_pauli(self.h, _p(x, C.c_uint64), _p(z, C.c_uint64), x.size, _p(out, C.c_double)))
+        return out[: x.size]
+
+    def expectation_value(self, observable) -> float:
+        """sum_k c_k <P_k> of an observable given as {label: coefficient} or
+        [(label, coefficient)], with Qiskit's Pauli labels (characters I, X,
+        Y, Z; the last character is qubit 0), as Statevector.expectation_value
+        of a SparsePauliOp (python/python/qevedo/simulator/qiskit.py:584-589)."""
+        labels, coeffs = pauli_terms(observable, self.num_qubits)
+        x, z = zip(*(pauli_masks(label) for label in labels)) if labels else ((), ())
+        values = self.expect_pauli(list(x), list(z))
+        return float(np.real(np.sum(np.asarray(coeffs) * values)))
 
     def measure(self, qubit: int, uniform: float) -> int:
         o = C.c_uint8()
