@@ -181,6 +181,19 @@ int qvd_sv_reset(qvd_sv *sv, uint32_t qubit, double uniform);
  * i-th smallest target; bit-exact with the reference's algorithm on the same
  * amplitudes and uniforms. */
 int qvd_sv_sample(const qvd_sv *sv, const double *uniforms, uint64_t shots, uint64_t *out);
+/* qvd_sv_sample without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The sampler's buffers (chunk sums, prefix, bounds, uniforms,
+ * sorted targets, outcomes, sort temporaries) come from one fixed 64 MiB
+ * device array of the library, which every GPU it runs on holds until the
+ * process ends. Requests of up to about 1.5 * 10^6 shots on states of up to
+ * 33 qubits fit; one whose buffers do not fit returns QVD_EINVAL and must use
+ * qvd_sv_sample. The array is
+ * shared by every handle on a GPU and keeps the last call's contents; calls
+ * from one process are serialised, so calls that would overlap on one GPU
+ * wait for each other. Same outcomes as qvd_sv_sample. */
+int qvd_sv_sample_noalloc(const qvd_sv *sv, const double *uniforms, uint64_t shots, uint64_t *out);
 
 /* Distance between two states of the same size (any precisions), in FP64
  * on the device: max_abs = max |a_i - b_i|, diff_sq = sum |a_i - b_i|^2,

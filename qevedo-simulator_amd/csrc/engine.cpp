@@ -1534,6 +1534,32 @@ int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, ui
     });
 }
 
+int qvd_sv_sample_noalloc(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, uint64_t *out) {
+    return guard([&] {
+        check_handle(sv_c);
+        qvd_sv *sv = const_cast<qvd_sv *>(sv_c);
+        if (shots == 0) return;
+        require(uniforms != nullptr && out != nullptr, "null uniforms or output");
+        require(shots < (uint64_t(1) << 31), "at most 2^31 - 1 shots per call");
+        sv->bind();
+        const double t0 = now();
+        SampleScratch w;
+        const size_t need = carve_sample(sv, shots, nullptr, w, nullptr, nullptr);
+        if (need > NOALLOC_SCRATCH_BYTES)
+            fail(QVD_EINVAL, "qvd_sv_sample_noalloc: " + std::to_string(shots) + " shots on " + std::to_string(sv->n) +
+                                 " qubits need " + std::to_string(need) + " bytes of workspace, more than its fixed " +
+                                 std::to_string(NOALLOC_SCRATCH_BYTES) + " (use qvd_sv_sample)");
+        void *ptr = nullptr;
+        QVD_HIP(noalloc_scratch(&ptr));
+        // Every call on this device shares the one workspace: calls from this
+        // process are serialised here (the call returns after its GPU work).
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lock(mu);
+        sample_with(sv, uniforms, shots, out, static_cast<char *>(ptr));
+        sv->last_sample_seconds = now() - t0;
+    });
+}
+
 int qvd_sv_compare(const qvd_sv *a_c, const qvd_sv *b, double *max_abs, double *diff_sq, double *ref_sq) {
     return guard([&] {
         check_handle(a_c);

@@ -25,6 +25,14 @@ struct SampleScratch {
     size_t sort_temp_bytes = 0;
 };
 
+// The fixed device workspace of the allocation-free sampling entry point
+// (qvd_sv_sample_noalloc): NOALLOC_SCRATCH_BYTES bytes of a file-scope
+// __device__ array, which every GPU the library runs on holds until the
+// process ends. *ptr receives its address on the current device, looked up
+// once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+hipError_t noalloc_scratch(void **ptr);
+
 // Pauli strings per pass of the expectation kernel (one X part, up to this
 // many Z parts).
 constexpr uint32_t PAULI_TERMS = 16;

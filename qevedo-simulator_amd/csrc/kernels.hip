@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -853,6 +855,29 @@ uint32_t expect_segment_bits(uint32_t n) { return n < EZ_TID_BITS + EZ_MAX_M_BIT
 template <typename T2>
 void launch_collapse(T2 *s, uint64_t len, uint32_t qubit, uint32_t outcome, double keep, hipStream_t st) {
     k_collapse<T2><<<grid_for(len, 256, 65536), 256, 0, st>>>(s, len, qubit, outcome, keep);
+}
+
+// Workspace of the allocation-free sampler. In host code `scratch` names a
+// host-side stand-in, not the device array: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(void **ptr) {
+    static std::mutex mu;
+    static void *addr[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
 }
 
 size_t sort_temp_bytes(uint64_t shots) {

@@ -29,7 +29,7 @@ EXPORTS = (
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
     "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
-    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_expect_pauli",
+    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_sample_noalloc", "qvd_sv_expect_pauli",
 )
 PLAN_VARIANTS, PLAN_FEATURES = 32, 24
 
@@ -120,6 +120,7 @@ def lib():
         L.qvd_sv_measure.argtypes = [vp, u32, d, P(C.c_uint8)]
         L.qvd_sv_reset.argtypes = [vp, u32, d]
         L.qvd_sv_sample.argtypes = [vp, P(d), u64, P(u64)]
+        L.qvd_sv_sample_noalloc.argtypes = [vp, P(d), u64, P(u64)]
         L.qvd_plan_ops.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(Stats)]
         L.qvd_sv_compare.argtypes = [vp, vp, P(d), P(d), P(d)]
         L.qvd_sv_last_sample_seconds.argtypes = [vp, P(d)]
