@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 
 #include "common.h"
@@ -85,6 +86,7 @@ struct qvd_sv {
     hipStream_t stream = nullptr;
     DBuf scratch;    // small reduction buffers
     DBuf sample_ws;  // sampling workspace, grown to the largest call and kept
+    DBuf plan_ws;    // a plan's uploaded programs, tables and matrices, grown to the largest plan and kept
     double last_sample_seconds = 0;
     std::vector<double> stage_seconds;  // register-tile stage kernels of the last timed apply_ops
 
@@ -215,8 +217,25 @@ int num_cus(int device) {
     return g_num_cus;
 }
 
+// The coefficients a full-state op's kernel reads: the diagonal of a
+// diagonal op, else the row-major matrix, in the state's precision.
 template <typename T2>
-void apply_full(qvd_sv *sv, T2 *s, const Op &op, bool valu_only = false) {
+std::vector<T2> full_coefficients(const Op &op) {
+    std::vector<T2> c;
+    if (op.diagonal) {
+        c.resize(op.m.dim());
+        for (size_t i = 0; i < c.size(); i++) c[i] = to_t2<T2>(op.m.at(i, i));
+    } else {
+        c.resize(op.m.a.size());
+        for (size_t i = 0; i < c.size(); i++) c[i] = to_t2<T2>(op.m.a[i]);
+    }
+    return c;
+}
+
+// One op as its own HBM pass, its coefficients (full_coefficients) already
+// in device memory at `coef`; enqueued on the handle's stream, not waited for.
+template <typename T2>
+void apply_full(qvd_sv *sv, T2 *s, const Op &op, const T2 *coef, bool valu_only = false) {
     const uint32_t k = (uint32_t)op.targets.size();
     if (k == 0) return;
     if (k > MAX_GATE_QUBITS)
@@ -227,15 +246,10 @@ void apply_full(qvd_sv *sv, T2 *s, const Op &op, bool valu_only = false) {
         if (c.second) cval |= uint64_t(1) << c.first;
     }
     if (op.diagonal) {
-        std::vector<T2> e(size_t(1) << k);
-        for (size_t i = 0; i < e.size(); i++) e[i] = to_t2<T2>(op.m.at(i, i));
-        DBuf dev(e.size() * sizeof(T2));
-        QVD_HIP(hipMemcpyAsync(dev.p, e.data(), dev.bytes, hipMemcpyHostToDevice, sv->stream));
         DiagTargets t{};
         for (uint32_t j = 0; j < k; j++) t.q[j] = op.targets[j];
-        launch_diag<T2>(s, sv->len, k, t, cmask, cval, dev.as<T2>(), sv->stream);
+        launch_diag<T2>(s, sv->len, k, t, cmask, cval, coef, sv->stream);
         QVD_HIP(hipGetLastError());
-        sv->sync();
         return;
     }
     GateDesc d{};
@@ -252,22 +266,39 @@ void apply_full(qvd_sv *sv, T2 *s, const Op &op, bool valu_only = false) {
     }
     d.ctrl_val = cval;
     d.iterations = uint64_t(1) << (sv->n - d.nins);
-    std::vector<T2> m(op.m.a.size());
-    for (size_t i = 0; i < m.size(); i++) m[i] = to_t2<T2>(op.m.a[i]);
-    DBuf dev(m.size() * sizeof(T2));
-    QVD_HIP(hipMemcpyAsync(dev.p, m.data(), dev.bytes, hipMemcpyHostToDevice, sv->stream));
     if constexpr (sizeof(T2) == sizeof(float2)) {
         // 5- and 6-qubit gates in complex64: FP32 matrix cores.
         if ((k == 5 || k == 6) && !valu_only) {
-            QVD_HIP(launch_dense_mfma(s, d, dev.as<float2>(), (unsigned)num_cus(sv->device), sv->stream));
-            sv->sync();
+            QVD_HIP(launch_dense_mfma(s, d, coef, (unsigned)num_cus(sv->device), sv->stream));
             return;
         }
     }
-    launch_dense<T2>(s, d, dev.as<T2>(), sv->stream);
+    launch_dense<T2>(s, d, coef, sv->stream);
     QVD_HIP(hipGetLastError());
-    sv->sync();
 }
+
+// Host data packed for one upload: each part 256-byte aligned.
+struct Upload {
+    std::vector<unsigned char> bytes;
+    size_t put(const void *p, size_t n) {
+        const size_t at = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(at + std::max<size_t>(n, 1));
+        if (n) std::memcpy(bytes.data() + at, p, n);
+        return at;
+    }
+    template <typename T> size_t put(const std::vector<T> &v) { return put(v.data(), v.size() * sizeof(T)); }
+    // Copies everything into the handle's plan arena (grown when needed) on
+    // its stream; returns the arena's base.
+    char *to_device(qvd_sv *sv) {
+        if (sv->plan_ws.bytes < bytes.size()) {
+            sv->sync();  // the arena may still feed a kernel of the previous call
+            sv->plan_ws.alloc(std::max(bytes.size(), sv->plan_ws.bytes * 2));
+        }
+        if (!bytes.empty())
+            QVD_HIP(hipMemcpyAsync(sv->plan_ws.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, sv->stream));
+        return sv->plan_ws.as<char>();
+    }
+};
 
 Op make_op(uint32_t n, const double *matrix, const uint32_t *targets, uint32_t k, const uint32_t *controls,
            const uint8_t *values, uint32_t c) {
@@ -413,32 +444,27 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
         lower_stage<T2>(step.stage, sv->n, sops, coefs, d);
         descs.push_back(d);
     }
-    DBuf dops(std::max<size_t>(sops.size(), 1) * sizeof(StageOp));
-    DBuf dcoef(std::max<size_t>(coefs.size(), 1) * sizeof(T2));
-    if (!sops.empty()) {
-        QVD_HIP(hipMemcpyAsync(dops.p, sops.data(), sops.size() * sizeof(StageOp), hipMemcpyHostToDevice, sv->stream));
-        QVD_HIP(hipMemcpyAsync(dcoef.p, coefs.data(), coefs.size() * sizeof(T2), hipMemcpyHostToDevice, sv->stream));
-    }
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
     inline_coefficients(vins, vco_t);
     inline_phase_records(vins, vints, vco_t);
-    DBuf dvins(std::max<size_t>(vins.size(), 1) * sizeof(VInstr));
-    DBuf dvco(std::max<size_t>(vco_t.size(), 1) * sizeof(T2));
-    DBuf dvlds(std::max<size_t>(vlds.size(), 1) * sizeof(StageOp));
-    DBuf dvints(std::max<size_t>(vints.size(), 1) * sizeof(uint32_t));
-    if (!vins.empty())
-        QVD_HIP(hipMemcpyAsync(dvins.p, vins.data(), vins.size() * sizeof(VInstr), hipMemcpyHostToDevice, sv->stream));
-    if (!vco_t.empty())
-        QVD_HIP(hipMemcpyAsync(dvco.p, vco_t.data(), vco_t.size() * sizeof(T2), hipMemcpyHostToDevice, sv->stream));
-    if (!vlds.empty())
-        QVD_HIP(hipMemcpyAsync(dvlds.p, vlds.data(), vlds.size() * sizeof(StageOp), hipMemcpyHostToDevice, sv->stream));
-    if (!vints.empty())
-        QVD_HIP(hipMemcpyAsync(dvints.p, vints.data(), vints.size() * sizeof(uint32_t), hipMemcpyHostToDevice, sv->stream));
-    DBuf dvdescs(std::max<size_t>(vdescs.size(), 1) * sizeof(VStageDesc));
-    if (!vdescs.empty())
-        QVD_HIP(hipMemcpyAsync(dvdescs.p, vdescs.data(), vdescs.size() * sizeof(VStageDesc), hipMemcpyHostToDevice,
-                               sv->stream));
+    // Everything the launches read goes up in one copy into the handle's
+    // plan arena (kept between calls), full-state ops' coefficients included.
+    Upload up;
+    const size_t o_ops = up.put(sops), o_coef = up.put(coefs), o_vins = up.put(vins), o_vco = up.put(vco_t);
+    const size_t o_vlds = up.put(vlds), o_vints = up.put(vints), o_vdescs = up.put(vdescs);
+    std::vector<size_t> o_full;
+    for (const Step &step : steps)
+        if (!step.is_stage) o_full.push_back(up.put(full_coefficients<T2>(step.op)));
+    char *base = up.to_device(sv);
+    const StageOp *dops = reinterpret_cast<const StageOp *>(base + o_ops);
+    const T2 *dcoef = reinterpret_cast<const T2 *>(base + o_coef);
+    const VInstr *dvins = reinterpret_cast<const VInstr *>(base + o_vins);
+    const T2 *dvco = reinterpret_cast<const T2 *>(base + o_vco);
+    const StageOp *dvlds = reinterpret_cast<const StageOp *>(base + o_vlds);
+    const uint32_t *dvints = reinterpret_cast<const uint32_t *>(base + o_vints);
+    const VStageDesc *dvdescs = reinterpret_cast<const VStageDesc *>(base + o_vdescs);
+    size_t fi = 0;
     const int cus = num_cus(sv->device);
     size_t si = 0, vi = 0;
     std::vector<char> reg_launch;  // per timed stage launch: register-tile or not
@@ -449,8 +475,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, (uint64_t)cus);
             timer.begin(timer.stage);
             reg_launch.push_back(1);
-            QVD_HIP(launch_vstage<T2>(s, dvdescs.as<VStageDesc>() + vi, 1, nullptr, dvins.as<VInstr>(), dvco.as<T2>(),
-                                      dvlds.as<StageOp>(), dvints.as<uint32_t>(), grid, sv->stream));
+            QVD_HIP(launch_vstage<T2>(s, dvdescs + vi, 1, nullptr, dvins, dvco, dvlds, dvints, grid, sv->stream));
             vi++;
             timer.end(timer.stage);
             st.stages++;
@@ -465,14 +490,15 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, want);
             timer.begin(timer.stage);
             reg_launch.push_back(0);
-            launch_stage<T2>(s, d, dops.as<StageOp>(), dcoef.as<T2>(), grid, sv->stream);
+            launch_stage<T2>(s, d, dops, dcoef, grid, sv->stream);
             QVD_HIP(hipGetLastError());
             timer.end(timer.stage);
             st.stages++;
             st.passes++;
         } else {
             timer.begin(timer.other);
-            apply_full<T2>(sv, s, step.op, (flags & QVD_FLAG_DENSE_VALU) != 0);
+            apply_full<T2>(sv, s, step.op, reinterpret_cast<const T2 *>(base + o_full[fi++]),
+                           (flags & QVD_FLAG_DENSE_VALU) != 0);
             timer.end(timer.other);
             st.kernels++;
             st.passes++;
@@ -1266,7 +1292,11 @@ int qvd_sv_apply(qvd_sv *sv, const double *matrix, const uint32_t *targets, uint
         Op op = make_op(sv->n, matrix, targets, num_targets, controls, control_values, num_controls);
         sv->bind();
         with_type(sv, [&](auto *s) {
-            apply_full(sv, s, op);
+            using T2 = std::remove_pointer_t<decltype(s)>;
+            Upload up;
+            const size_t o = up.put(full_coefficients<T2>(op));
+            apply_full(sv, s, op, reinterpret_cast<const T2 *>(up.to_device(sv) + o));
+            sv->sync();
             return 0;
         });
     });
@@ -1457,15 +1487,16 @@ int qvd_sv_expect_z(const qvd_sv *sv_c, double *z, double *zz) {
         sv->bind();
         const uint32_t B = expect_segment_bits(n);
         const uint64_t segs = uint64_t(1) << (n - B);
-        DBuf seg(segs * 2 * B * sizeof(double));
-        DBuf res((2 * n) * sizeof(double));
+        // segment sums, then the results, in the handle's reduction scratch
+        double *seg = sv->scratch_doubles(segs * 2 * B + 2 * n);
+        double *res = seg + segs * 2 * B;
         with_type(sv, [&](auto *s) {
-            launch_expect_z(s, n, seg.as<double>(), res.as<double>(), res.as<double>() + n, sv->stream);
+            launch_expect_z(s, n, seg, res, res + n, sv->stream);
             return 0;
         });
         QVD_HIP(hipGetLastError());
         std::vector<double> h(2 * n);
-        QVD_HIP(hipMemcpyAsync(h.data(), res.p, (2 * n - 1) * sizeof(double), hipMemcpyDeviceToHost, sv->stream));
+        QVD_HIP(hipMemcpyAsync(h.data(), res, (2 * n - 1) * sizeof(double), hipMemcpyDeviceToHost, sv->stream));
         sv->sync();
         std::copy(h.begin(), h.begin() + n, z);
         if (zz) std::copy(h.begin() + n, h.begin() + 2 * n - 1, zz);
@@ -1508,7 +1539,11 @@ int qvd_sv_reset(qvd_sv *sv, uint32_t qubit, double uniform) {
             x.m.at(1, 0) = C64(1, 0);
             x.diagonal = false;
             with_type(sv, [&](auto *s) {
-                apply_full(sv, s, x);
+                using T2 = std::remove_pointer_t<decltype(s)>;
+                Upload up;
+                const size_t o = up.put(full_coefficients<T2>(x));
+                apply_full(sv, s, x, reinterpret_cast<const T2 *>(up.to_device(sv) + o));
+                sv->sync();
                 return 0;
             });
         }
