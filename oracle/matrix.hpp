@@ -176,11 +176,21 @@ inline Matrix base_matrix(const std::string &n, const std::vector<double> &p) {
         return Matrix::from({C64(c, 0), C64(-s, 0), C64(s, 0), C64(c, 0)});
     }
     if (n == "rz") { need(1); return Matrix::diagonal({phase(-p[0] / 2), phase(p[0] / 2)}); }
-    if (n == "p" || n == "u1") { need(1); return Matrix::diagonal({one, phase(p[0])}); }
-    if (n == "u" || n == "u3") {
+    if (n == "p" || n == "u1" || n == "phase") { need(1); return Matrix::diagonal({one, phase(p[0])}); }
+    if (n == "u" || n == "u3" || n == "U") {
         need(3);
         double s = std::sin(p[0] / 2), c = std::cos(p[0] / 2);
         return Matrix::from({C64(c, 0), -phase(p[2]) * s, phase(p[1]) * s, phase(p[1] + p[2]) * c});
+    }
+    // qasm.rs:1394-1425: u0 (identity, one ignored parameter), u2 = U(pi/2, .),
+    // and the base of cu: e^(i gamma) U(theta, phi, lambda)
+    if (n == "u0") { need(1); return Matrix::identity(1); }
+    if (n == "u2") { need(2); return base_matrix("u", {M_PI / 2.0, p[0], p[1]}); }
+    if (n == "u_gamma") {
+        need(4);
+        Matrix m = base_matrix("u", {p[0], p[1], p[2]});
+        for (C64 &x : m.data) x = cmul(x, phase(p[3]));
+        return m;
     }
     if (n == "swap") {
         need(0);
@@ -206,6 +216,11 @@ inline bool controlled_form(const std::string &n, unsigned &controls, std::strin
         {"cx", 1, "x"},   {"cy", 1, "y"},   {"cz", 1, "z"},     {"ch", 1, "h"},
         {"cp", 1, "p"},   {"cu1", 1, "p"},  {"crx", 1, "rx"},   {"cry", 1, "ry"},
         {"crz", 1, "rz"}, {"ccx", 2, "x"},  {"cswap", 1, "swap"},
+        // aliases and the controlled gates of the reference's QASM reader
+        // (qasm.rs:1394-1435), controls first
+        {"CX", 1, "x"},   {"cnot", 1, "x"}, {"toffoli", 2, "x"}, {"fredkin", 1, "swap"},
+        {"cphase", 1, "p"}, {"csx", 1, "sx"}, {"cu3", 1, "u3"}, {"cu", 1, "u_gamma"},
+        {"c3x", 3, "x"},  {"c4x", 4, "x"},  {"c3sqrtx", 3, "sx"},
     };
     for (auto &e : table)
         if (n == e.name) { controls = e.c; base = e.base; return true; }

@@ -117,7 +117,7 @@ def deposit(value: int, mask: int) -> int:
 
 def gate_matrix(name: str, params=()) -> np.ndarray:
     p = np.ascontiguousarray(params, dtype=np.float64)
-    out = np.zeros(2 * 64, dtype=np.float64)
+    out = np.zeros(2 * 1024, dtype=np.float64)  # up to 5 qubits (c4x)
     k = C.c_uint32()
     _check(lib().orc_gate_matrix(name.encode(), _p(p, C.c_double), len(p), _p(out, C.c_double), len(out) // 2, C.byref(k)))
     dim = 1 << k.value

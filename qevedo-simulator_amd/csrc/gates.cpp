@@ -68,11 +68,22 @@ Lowered lower(const std::string &name, const double *p, uint32_t np) {
         }
         if (n == "rz") return diag({polar1(-q[0] / 2), polar1(q[0] / 2)});
         if (n == "p") return diag({one, polar1(q[0])});
-        if (n == "u") {
-            const double s = std::sin(q[0] / 2), c = std::cos(q[0] / 2);
-            const C64 l = polar1(q[2]), f = polar1(q[1]), fl = polar1(q[1] + q[2]);
+        auto u3 = [&](double theta, double phi, double lam) {
+            const double s = std::sin(theta / 2), c = std::cos(theta / 2);
+            const C64 l = polar1(lam), f = polar1(phi), fl = polar1(phi + lam);
             return m2(C64(c, 0), C64(-l.real() * s, -l.imag() * s), C64(f.real() * s, f.imag() * s),
                       C64(fl.real() * c, fl.imag() * c));
+        };
+        if (n == "u") return u3(q[0], q[1], q[2]);
+        // u2(phi, lambda) = U(pi/2, phi, lambda) (src/qasm.rs:1405)
+        if (n == "u2") return u3(M_PI / 2.0, q[0], q[1]);
+        // the base of cu(theta, phi, lambda, gamma): e^(i gamma) U(theta, phi, lambda)
+        // (src/qasm.rs:1420-1425)
+        if (n == "cu") {
+            HMatrix m = u3(q[0], q[1], q[2]);
+            const C64 g = polar1(q[3]);
+            for (C64 &x : m.a) x = cmul(x, g);
+            return m;
         }
         if (n == "swap") {
             HMatrix m(2);
@@ -110,6 +121,13 @@ Lowered lower(const std::string &name, const double *p, uint32_t np) {
         {"ch", 1, "h", 0},    {"cp", 1, "p", 1},     {"cu1", 1, "p", 1},     {"cphase", 1, "p", 1},
         {"crx", 1, "rx", 1},  {"cry", 1, "ry", 1},   {"crz", 1, "rz", 1},    {"ccx", 2, "x", 0},
         {"toffoli", 2, "x", 0}, {"cswap", 1, "swap", 0}, {"fredkin", 1, "swap", 0},
+        // the OpenQASM primitives and the qelib1 / stdgates names the
+        // reference's reader maps (src/qasm.rs:1394-1435): u0 is the
+        // identity (its parameter is a duration), the rest are their base
+        // gates with controls first
+        {"U", 0, "u", 3},     {"CX", 1, "x", 0},     {"u0", 0, "id", 1},     {"u2", 0, "u2", 2},
+        {"csx", 1, "sx", 0},  {"cu3", 1, "u", 3},    {"cu", 1, "cu", 4},     {"c3x", 3, "x", 0},
+        {"c4x", 4, "x", 0},   {"c3sqrtx", 3, "sx", 0},
     };
     for (const Entry &e : table)
         if (name == e.name) {

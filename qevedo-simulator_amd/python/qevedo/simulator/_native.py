@@ -178,9 +178,9 @@ def state_vector_fits(num_qubits: int, precision: str = "single") -> tuple:
 
 def gate_matrix(name: str, params=()) -> np.ndarray:
     p = np.ascontiguousarray(params, dtype=np.float64)
-    out = np.zeros(2 * 64, dtype=np.float64)
+    out = np.zeros(2 * 1024, dtype=np.float64)  # up to 5 qubits (c4x)
     k = C.c_uint32()
-    _check(lib().qvd_gate_matrix(name.encode(), _p(p, C.c_double), p.size, _p(out, C.c_double), 64, C.byref(k)))
+    _check(lib().qvd_gate_matrix(name.encode(), _p(p, C.c_double), p.size, _p(out, C.c_double), 1024, C.byref(k)))
     dim = 1 << k.value
     return out[: 2 * dim * dim].view(np.complex128).reshape(dim, dim).copy()
 

@@ -210,9 +210,9 @@ class Circuit:
 
     @staticmethod
     def from_qasm(source: str) -> "Circuit":
-        """Read the OpenQASM 2 subset to_qasm writes: one qreg and one creg,
-        qelib1 gates with numeric or pi-expression parameters, measure, reset
-        and barrier."""
+        """Read an OpenQASM 2 or 3 program (see qevedo.simulator.qasm):
+        registers, qelib1 / stdgates gates, gate definitions, modifiers,
+        measure, reset, barrier and Pauli feedback under `if`."""
         from .qasm import parse
 
         return parse(source)

@@ -27,7 +27,10 @@ GATES = [("id", ()), ("h", ()), ("x", ()), ("y", ()), ("z", ()), ("s", ()), ("sd
          ("tdg", ()), ("sx", ()), ("sxdg", ()), ("rx", (0.3,)), ("ry", (0.4,)), ("rz", (0.5,)),
          ("p", (0.6,)), ("u1", (0.6,)), ("u3", (0.1, 0.2, 0.3)), ("cx", ()), ("cy", ()), ("cz", ()),
          ("ch", ()), ("cp", (0.7,)), ("cu1", (0.7,)), ("crx", (0.8,)), ("cry", (0.9,)), ("crz", (1.0,)),
-         ("swap", ()), ("rzz", (1.1,)), ("rxx", (1.2,)), ("ccx", ()), ("cswap", ())]
+         ("swap", ()), ("rzz", (1.1,)), ("rxx", (1.2,)), ("ccx", ()), ("cswap", ()), ("U", (0.1, 0.2, 0.3)),
+         ("CX", ()), ("u0", (0.5,)), ("u2", (0.1, 0.2)), ("csx", ()), ("cu3", (0.1, 0.2, 0.3)),
+         ("cu", (0.1, 0.2, 0.3, 0.4)), ("c3x", ()), ("c4x", ()), ("c3sqrtx", ()), ("cnot", ()), ("toffoli", ()),
+         ("fredkin", ()), ("cphase", (0.3,)), ("phase", (0.3,))]
 
 
 @pytest.mark.parametrize("name,params", GATES)

@@ -141,3 +141,39 @@ def test_sampling_after_relabeling_keys():
             s.apply_ops(lower_gates(gates), 4, tile)
             keys = sorted(_count(s.sample(u), [(q, q) for q in range(10)], 10))
         assert keys == ["1010000000", "1110000001"], tile
+
+
+@pytest.mark.gpu
+def test_teleportation_with_if_statements():
+    """tests/qasm.rs:666-698 on the state vector: |+i> teleported from q[0]
+    to q[2] with OpenQASM 3 `if`s (classical feedback), the preparation
+    undone: q[2] reads 0 in every shot, and the four outcomes of the two
+    measured qubits all occur."""
+    program = ('OPENQASM 3.0;\ninclude "stdgates.inc";\nqubit[3] q;\nbit[3] c;\n'
+               "h q[0]; s q[0];\nh q[1]; cx q[1], q[2];\ncx q[0], q[1]; h q[0];\n"
+               "c[0] = measure q[0];\nc[1] = measure q[1];\n"
+               "if (c[1]) x q[2];\nif (c[0]) z q[2];\nsdg q[2]; h q[2];\nc[2] = measure q[2];\n")
+    counts = run(Circuit.from_qasm(program), 400, seed=3).counts
+    assert len(counts) == 4 and all(k.startswith("0") for k in counts), counts
+
+
+@pytest.mark.gpu
+def test_qasm_programs_with_modifiers_run_on_the_engine():
+    """Programs whose gates become unitaries (modifiers, definitions, c3x,
+    cu) run through the engine and match the oracle's FP64 applier."""
+    from helpers import random_unitary, rng
+    from oracle import oracle
+
+    program = ('OPENQASM 3.0;\ninclude "stdgates.inc";\n'
+               "gate g(a) x, y { rx(a) x; cz x, y; ry(a / 3) y; }\nqubit[5] q;\n"
+               "h q; ctrl @ inv @ g(0.7) q[2], q[0], q[1];\nnegctrl(2) @ ry(1.1) q[3], q[4], q[0];\n"
+               "c3x q[0], q[1], q[2], q[4];\ncu(0.3, 0.2, 0.1, 0.5) q[4], q[3];\npow(3) @ sx q[1];\n"
+               "rccx q[1], q[3], q[2];\n")
+    c = Circuit.from_qasm(program)
+    want = np.zeros(32, dtype=np.complex128)
+    want[0] = 1
+    for kind, name, qubits, params in c._ins:
+        m = oracle.gate_matrix(name, params) if kind == "gate" else name
+        oracle.reference_apply(want, m, list(qubits))
+    got = statevector(c)
+    assert np.max(np.abs(got - want)) < 1e-12
