@@ -127,6 +127,11 @@ int qvd_sv_precision(const qvd_sv *sv, int *out);
 /* Reset to |0...0> without reallocating. */
 int qvd_sv_set_zero(qvd_sv *sv);
 
+/* A copy of src's amplitudes in dst (same size and precision), device to
+ * device: the branch copies of the batched shot loop (the reference re-runs
+ * every shot from |0...0>, src/simulator.rs:383-425). */
+int qvd_sv_copy(qvd_sv *dst, const qvd_sv *src);
+
 /* StateVector::from_amplitudes / set (src/state.rs:52-63, 137-143):
  * `amplitudes` holds 2^n complex128 values in logical order. */
 int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len);

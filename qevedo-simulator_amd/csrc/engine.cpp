@@ -1243,6 +1243,21 @@ int qvd_sv_set_zero(qvd_sv *sv) {
     });
 }
 
+int qvd_sv_copy(qvd_sv *dst, const qvd_sv *src) {
+    return guard([&] {
+        check_handle(dst);
+        check_handle(src);
+        require(dst->n == src->n && dst->precision == src->precision, "states of different sizes or precisions");
+        require(dst->device == src->device, "states on different devices");
+        if (dst == src) return;
+        src->sync();
+        dst->bind();
+        QVD_HIP(hipMemcpyAsync(dst->amps.p, src->amps.p, src->len * src->elem(), hipMemcpyDeviceToDevice,
+                               dst->stream));
+        dst->sync();
+    });
+}
+
 int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len) {
     return guard([&] {
         check_handle(sv);

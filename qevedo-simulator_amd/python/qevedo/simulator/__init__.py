@@ -125,8 +125,10 @@ def run(circuit, shots: int = 1024, *, backend: str = "auto", seed: int | None =
     """Sample shots of a circuit on the state vector (simulator.rs:291-427).
 
     Circuits whose measurements all come at the end are simulated once and
-    sampled with the two-pass sampler; mid-circuit measurements, resets and
-    feedback run shot by shot. ``max_bond_dimension``, ``truncation_threshold``
+    sampled with the two-pass sampler; with mid-circuit measurements, resets
+    and feedback, shots that share a measurement history share one state
+    (see _run_branches), with the counts of the reference's shot-by-shot
+    loop. ``max_bond_dimension``, ``truncation_threshold``
     and ``threads`` belong to other backends and CPU placement and are
     accepted for compatibility only.
     """
@@ -148,44 +150,139 @@ def run(circuit, shots: int = 1024, *, backend: str = "auto", seed: int | None =
             counts = _count(outcomes, measures, c.num_clbits)
             measure_seconds = time.perf_counter() - t0
         return RunResult(counts, st["gates"], st["gate_seconds"], measure_seconds, st["kernels"], st["passes"])
+    draws = sum(1 for k, *_ in ins if k in ("measure", "reset"))
+    # One uniform per measure or reset, shot after shot: the order in which
+    # the reference's shot loop draws them (simulator.rs:383-425).
+    u = rng.random(shots * draws).reshape(shots, draws) if draws else np.zeros((shots, 0))
+    _, state_bytes, device_bytes = _native.state_vector_fits(c.num_qubits, precision)
+    if (draws + 2) * state_bytes <= 0.5 * device_bytes:
+        return _run_branches(c, u, precision)
+    return _run_shot_by_shot(c, u, precision)
+
+
+class _Stats:
+    def __init__(self):
+        self.gates = self.kernels = self.passes = 0
+        self.gate_s = self.meas_s = 0.0
+
+    def add(self, st):
+        self.gates += st["gates"]
+        self.kernels += st["kernels"]
+        self.passes += st["passes"]
+        self.gate_s += st["gate_seconds"]
+
+
+def _flush(c, sv, pending, stats):
+    if pending:
+        stats.add(sv.apply_ops(c.ops(pending)))
+        pending.clear()
+
+
+def _run_branches(c, u, precision) -> RunResult:
+    """The shot loop with shots that share a measurement history sharing one
+    state: a depth-first walk over the outcome tree. Each branch holds the
+    shots whose outcomes so far agree; at a measure or reset the branch's
+    P(1) is computed once, its shots split by their own uniform (outcome =
+    u < P(1), as measure.rs:68-80 decides per shot), and a branch that
+    splits continues in a device copy of its state for the shots with
+    outcome 1. Every branch state is bit for bit the state each of its
+    shots reaches alone, so the counts equal the shot-by-shot loop's; the
+    gates run once per branch instead of once per shot."""
+    ins = c._ins
+    shots = u.shape[0]
     counts: dict = {}
-    gates = kernels = passes = 0
-    gate_s = meas_s = 0.0
-    with StateVector(c.num_qubits, precision) as sv:
-        for _ in range(shots):
-            sv.set_zero()
-            clbits = [0] * c.num_clbits
-            pending = []
-
-            def flush():
-                nonlocal gates, kernels, passes, gate_s
-                if pending:
-                    st = sv.apply_ops(c.ops(pending))
-                    gates += st["gates"]
-                    kernels += st["kernels"]
-                    passes += st["passes"]
-                    gate_s += st["gate_seconds"]
-                    pending.clear()
-
-            for ins_ in ins:
-                kind, name, qubits, params = ins_
+    stats = _Stats()
+    root = StateVector(c.num_qubits, precision)
+    # (state, shot indices, next instruction, classical bits, next draw, pending gates)
+    stack = [(root, np.arange(shots), 0, [0] * c.num_clbits, 0, [])]
+    try:
+        while stack:
+            sv, idx, i, clbits, k, pending = stack.pop()
+            while i < len(ins):
+                kind, name, qubits, params = ins[i]
+                i += 1
                 if kind in ("gate", "unitary"):
-                    pending.append(ins_)
-                elif kind == "measure":
-                    flush()
+                    pending.append(ins[i - 1])
+                elif kind in ("measure", "reset"):
+                    _flush(c, sv, pending, stats)
                     t0 = time.perf_counter()
-                    clbits[params[0]] = sv.measure(qubits[0], rng.random())
-                    meas_s += time.perf_counter() - t0
-                elif kind == "reset":
-                    flush()
-                    sv.reset(qubits[0], rng.random())
+                    q = qubits[0]
+                    p1 = min(max(sv.probability_one(q), 0.0), 1.0)
+                    one = u[idx, k] < p1
+                    ones, zeros = idx[one], idx[~one]
+                    k += 1
+                    def collapse(state, outcome):
+                        # a uniform of -1 is below any P(1) > 0 (some shot's u
+                        # is below P(1) when outcome 1 occurs), 2 above any
+                        f = state.measure if kind == "measure" else state.reset
+                        f(q, -1.0 if outcome else 2.0)
+
+                    if ones.size and zeros.size:
+                        other = StateVector(c.num_qubits, precision)
+                        other.copy_from(sv)
+                        collapse(sv, 1)
+                        bits = list(clbits)
+                        if kind == "measure":
+                            bits[params[0]] = 1
+                        stack.append((sv, ones, i, bits, k, []))
+                        sv, idx = other, zeros
+                        collapse(sv, 0)
+                        if kind == "measure":
+                            clbits = list(clbits)
+                            clbits[params[0]] = 0
+                    else:
+                        outcome = 1 if ones.size else 0
+                        collapse(sv, outcome)
+                        if kind == "measure":
+                            clbits = list(clbits)
+                            clbits[params[0]] = outcome
+                    stats.meas_s += time.perf_counter() - t0
                 elif kind == "feedback":
                     if clbits[params[0]]:
                         pending.append(("gate", name, qubits, ()))
-            flush()
-            key = _bitstring(sum(b << i for i, b in enumerate(clbits)), c.num_clbits)
+            key = _bitstring(sum(b << j for j, b in enumerate(clbits)), c.num_clbits)
+            counts[key] = counts.get(key, 0) + int(idx.size)
+            sv.close()
+    finally:
+        for entry in stack:
+            entry[0].close()
+    return RunResult(dict(sorted(counts.items())), stats.gates, stats.gate_s, stats.meas_s, stats.kernels,
+                     stats.passes)
+
+
+def _run_shot_by_shot(c, u, precision) -> RunResult:
+    """The reference's shot loop (simulator.rs:383-425), one shot after
+    another on one state, with the same uniforms; used when the branch
+    copies of _run_branches would not fit in device memory."""
+    ins = c._ins
+    counts: dict = {}
+    stats = _Stats()
+    with StateVector(c.num_qubits, precision) as sv:
+        for s in range(u.shape[0]):
+            sv.set_zero()
+            clbits = [0] * c.num_clbits
+            pending = []
+            k = 0
+            for kind, name, qubits, params in ins:
+                if kind in ("gate", "unitary"):
+                    pending.append((kind, name, qubits, params))
+                elif kind == "measure":
+                    _flush(c, sv, pending, stats)
+                    t0 = time.perf_counter()
+                    clbits[params[0]] = sv.measure(qubits[0], float(u[s, k]))
+                    stats.meas_s += time.perf_counter() - t0
+                    k += 1
+                elif kind == "reset":
+                    _flush(c, sv, pending, stats)
+                    sv.reset(qubits[0], float(u[s, k]))
+                    k += 1
+                elif kind == "feedback":
+                    if clbits[params[0]]:
+                        pending.append(("gate", name, qubits, ()))
+            key = _bitstring(sum(b << j for j, b in enumerate(clbits)), c.num_clbits)
             counts[key] = counts.get(key, 0) + 1
-    return RunResult(dict(sorted(counts.items())), gates, gate_s, meas_s, kernels, passes)
+    return RunResult(dict(sorted(counts.items())), stats.gates, stats.gate_s, stats.meas_s, stats.kernels,
+                     stats.passes)
 
 
 def statevector(circuit, *, threads=None) -> np.ndarray:

@@ -29,7 +29,7 @@ EXPORTS = (
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
     "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
-    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_sample_noalloc", "qvd_sv_expect_pauli",
+    "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_sample_noalloc", "qvd_sv_expect_pauli", "qvd_sv_copy",
 )
 PLAN_VARIANTS, PLAN_FEATURES = 32, 24
 
@@ -107,6 +107,7 @@ def lib():
         L.qvd_sv_num_qubits.argtypes = [vp, P(u32)]
         L.qvd_sv_precision.argtypes = [vp, P(C.c_int)]
         L.qvd_sv_set_zero.argtypes = [vp]
+        L.qvd_sv_copy.argtypes = [vp, vp]
         L.qvd_sv_set_amplitudes.argtypes = [vp, P(d), u64]
         L.qvd_sv_get_amplitudes.argtypes = [vp, P(d), u64]
         L.qvd_sv_apply.argtypes = [vp, P(d), P(u32), u32, P(u32), P(C.c_uint8), u32]
@@ -329,6 +330,10 @@ class StateVector:
 
     def set_zero(self) -> None:
         _check(lib().qvd_sv_set_zero(self.h))
+
+    def copy_from(self, other: "StateVector") -> None:
+        """This state becomes a copy of `other` (same size and precision), on the device."""
+        _check(lib().qvd_sv_copy(self.h, other.h))
 
     def set_amplitudes(self, amps) -> None:
         a = np.ascontiguousarray(amps, dtype=np.complex128)

@@ -177,3 +177,48 @@ def test_qasm_programs_with_modifiers_run_on_the_engine():
         oracle.reference_apply(want, m, list(qubits))
     got = statevector(c)
     assert np.max(np.abs(got - want)) < 1e-12
+
+
+def _dynamic_circuit(n, seed, steps=30):
+    """Random gates with mid-circuit measures, resets and Pauli feedback."""
+    r = random.Random(seed)
+    c = Circuit(n, n)
+    for _ in range(steps):
+        k = r.randrange(8)
+        a, b = r.sample(range(n), 2)
+        if k == 0:
+            c.measure(a, r.randrange(n))
+        elif k == 1:
+            c.reset(a)
+        elif k == 2:
+            c.feedback(r.choice("XYZ"), a, r.randrange(n))
+        elif k == 3:
+            c.cx(a, b)
+        elif k == 4:
+            c.rx(r.uniform(0, 6.28), a)
+        elif k == 5:
+            c.ry(r.uniform(0, 6.28), a)
+        elif k == 6:
+            c.h(a)
+        else:
+            c.cp(r.uniform(0, 6.28), a, b)
+    return c.measure_all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,seed,shots,precision", [(3, 1, 500, "single"), (6, 2, 400, "double"),
+                                                    (9, 3, 300, "single"), (15, 4, 200, "single")])
+def test_branch_walk_counts_equal_the_shot_by_shot_loop(n, seed, shots, precision):
+    """Shots sharing a measurement history share one state: the counts are
+    exactly those of running every shot alone with the same uniforms."""
+    from qevedo.simulator import _run_branches, _run_shot_by_shot
+
+    c = _dynamic_circuit(n, seed)
+    draws = sum(1 for k, *_ in c._ins if k in ("measure", "reset"))
+    u = np.random.Generator(np.random.PCG64(seed)).random(shots * draws).reshape(shots, draws)
+    branched = _run_branches(c, u, precision)
+    serial = _run_shot_by_shot(c, u, precision)
+    assert branched.counts == serial.counts
+    assert sum(branched.counts.values()) == shots
+    # the branch walk runs each gate segment once per branch, not per shot
+    assert branched.gates < serial.gates
