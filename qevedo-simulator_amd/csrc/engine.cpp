@@ -721,6 +721,9 @@ uint64_t ops_hash(uint32_t n, int precision, const qvd_options &opt, const qvd_o
     h = fnv(h, &precision, sizeof precision);
     h = fnv(h, &opt.max_fused_qubits, sizeof opt.max_fused_qubits);
     h = fnv(h, &opt.tile_bits, sizeof opt.tile_bits);
+    // the candidate set: every variant, or those equal to the base variant's plan
+    h = fnv(h, &opt.planner, sizeof opt.planner);
+    h = fnv(h, &opt.variant, sizeof opt.variant);
     for (uint64_t i = 0; i < num_ops; i++) {
         const qvd_op &o = ops[i];
         h = fnv(h, &o.num_targets, sizeof o.num_targets);

@@ -152,22 +152,51 @@ def test_learned_planner_falls_back_and_chooses_the_predicted_best():
 
 
 @pytest.mark.skipif(not os.path.exists(learned.DEFAULT_MODEL), reason="no trained planner model in the tree")
-def test_trained_model_plan_is_not_slower():
+def test_trained_model_regret_against_every_variant():
+    """On a held-out circuit (26-qubit kicked Ising, a theta and depth not in
+    the training set) every distinct plan variant is timed; the trained
+    model's choice over all variants is within 10 % of the fastest, and the
+    identity-preserving learned planner reproduces the heuristic's amplitudes
+    bit for bit."""
     model = learned.load()
     net = learned.install(model)
     n = 26
-    gates = library.kicked_ising(n, library.heavy_hex_edges(n), 8, 0.45)
+    gates = library.kicked_ising(n, library.heavy_hex_edges(n), 7, 0.37)
     ops = _native.OpArray(lower_gates(gates))
     s = StateVector(n, "single")
-    times = {}
-    for planner in (_native.PLANNER_HEURISTIC, _native.PLANNER_LEARNED, _native.PLANNER_HEURISTIC,
-                    _native.PLANNER_LEARNED):
-        s.set_zero()
-        st = s.apply_ops(ops, planner=planner, timing=True)
-        times.setdefault(planner, []).append(st["stage_kernel_seconds"] + st["other_kernel_seconds"])
-    h, l = min(times[_native.PLANNER_HEURISTIC]), min(times[_native.PLANNER_LEARNED])
-    print(f"heuristic {h * 1e3:.2f} ms, learned {l * 1e3:.2f} ms", flush=True)
-    assert l <= 1.05 * h
+
+    def timed(**kw):
+        best = None
+        for _ in range(3):
+            s.set_zero()
+            st = s.apply_ops(ops, timing=True, **kw)
+            t = st["stage_kernel_seconds"] + st["other_kernel_seconds"]
+            best = t if best is None else min(best, t)
+        return best, st["plan_variant"]
+
+    times, seen = {}, {}
+    for v in range(_native.PLAN_VARIANTS):
+        key = _native.plan_features(n, ops, variant=v).tobytes()
+        if key in seen:
+            times[v] = times[seen[key]]
+            continue
+        seen[key] = v
+        times[v] = timed(variant=v)[0]
+    fastest = min(times.values())
+    t_any, v_any = timed(planner=_native.PLANNER_LEARNED_ANY)
+    t_id, v_id = timed(planner=_native.PLANNER_LEARNED)
+    print(f"fastest variant {min(times, key=times.get)} {fastest * 1e3:.2f} ms; heuristic {times[0] * 1e3:.2f} ms; "
+          f"lookahead (24) {times[24] * 1e3:.2f} ms; learned over every variant: variant {v_any} "
+          f"{t_any * 1e3:.2f} ms (regret {times[v_any] / fastest:.3f}); identity-preserving: variant {v_id}",
+          flush=True)
+    assert times[v_any] <= 1.10 * fastest
+    assert v_id in _native.equivalent_variants(n, ops)
+    s.set_zero()
+    s.apply_ops(ops)
+    heuristic = s.amplitudes()
+    s.set_zero()
+    s.apply_ops(ops, planner=_native.PLANNER_LEARNED)
+    assert np.array_equal(s.amplitudes(), heuristic)
     _native.clear_planner_model()
     net.close()
     s.close()
