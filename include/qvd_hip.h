@@ -91,6 +91,13 @@ typedef struct qvd_options {
 /* qvd_options.flags: apply 5- and 6-qubit dense ops in complex64 with VALU
  * FMAs instead of the FP32 matrix cores (for comparisons). */
 #define QVD_FLAG_DENSE_VALU 2u
+/* qvd_options.flags for qvd_batch_create: pick each circuit's register-tile
+ * geometry (qvd_circuit.fixed_lanes left 0) by timing. Circuits of the same
+ * size and gate layout form a class; each class is run as a batch of its own
+ * (up to enough circuits to fill the device a few times) with every candidate
+ * geometry, and the fastest is kept, for this batch and for later batches of
+ * the process with that class. New in this engine. */
+#define QVD_FLAG_AUTOTUNE 4u
 
 /* What a run did: simulator::Stats (src/simulator.rs:70-93). */
 typedef struct qvd_stats {
@@ -266,10 +273,13 @@ int qvd_planner_set_model(const qvd_mlp *mlp, const float *mean, const float *sc
 
 /* ---- Batches of circuits ------------------------------------------------
  * Many circuits, each on its own state, run together: the engine plans every
- * circuit into register-tile stages, then runs stage level r (stage r of every
- * circuit) in one kernel launch (one descriptor per circuit). Reset, the levels and
- * the expectation values are captured once in a HIP graph at creation and
- * replayed by every qvd_batch_run. States smaller than one register tile
+ * circuit into register-tile stages and runs all of them in one launch of the
+ * stage kernel, whose workgroups take runs of tiles from a queue (stage r of
+ * every circuit before stage r + 1 of any) and start a circuit's stage once
+ * its previous stage has finished; the first stage starts from |0...0>
+ * without reading the state. The launch and the expectation values are
+ * captured once in a HIP graph at creation and replayed by every
+ * qvd_batch_run. States smaller than one register tile
  * (14 qubits in complex64, 13 in complex128) are widened with ancilla qubits
  * in |0>, which no op touches. New in this engine: the reference runs such
  * circuits one at a time (run_on, src/simulator.rs:355-426); the estimator
@@ -280,6 +290,12 @@ typedef struct qvd_circuit {
     uint32_t num_qubits;
     const qvd_op *ops;
     uint64_t num_ops;
+    /* Register-tile geometry of this circuit's stages: the number F of lane
+     * bits pinned to qubits 0..F-1 (each access then reads 2^F consecutive
+     * amplitudes; the other 6 - F lane bits hold any tile qubit). 0 = the
+     * default (4 in complex64, 3 in complex128) or, with QVD_FLAG_AUTOTUNE,
+     * the autotuner's choice; otherwise the default - 1 .. the default + 1. */
+    uint32_t fixed_lanes;
 } qvd_circuit;
 
 /* Plans, uploads and captures `count` circuits. Every op must plan into
@@ -292,11 +308,21 @@ int qvd_batch_destroy(qvd_batch *batch);
 /* Runs every circuit from |0...0> and writes <Z_q> (num_qubits values per
  * circuit) to z and <Z_q Z_{q+1}> (num_qubits - 1 per circuit) to zz, both
  * concatenated in circuit order, FP64. stats: gates, kernels, passes (stage
- * levels), stages (register-tile stage descriptors), gate_seconds (the graph
- * replay, host clock). */
+ * levels: the most stages of any circuit), stages (register-tile stage
+ * descriptors), gate_seconds (the graph replay, host clock). Fails with
+ * QVD_EHIP if a stage waited more than 10 s for its previous one. */
 int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats);
 /* Sizes of the z and zz arrays qvd_batch_run fills. */
 int qvd_batch_outputs(const qvd_batch *batch, uint64_t *num_z, uint64_t *num_zz);
+/* Copies circuit `index`'s state after the last qvd_batch_run (2^num_qubits
+ * amplitudes in the batch's precision, interleaved re/im) to host memory
+ * `out` of `capacity` amplitudes. No reference counterpart: for callers that
+ * need the states themselves, and for checking a batch against single
+ * states amplitude by amplitude. */
+int qvd_batch_amplitudes(const qvd_batch *batch, uint32_t index, void *out, uint64_t capacity);
+/* The register-tile geometry (fixed lane bits) each circuit was built with,
+ * `count` values into out. No reference counterpart. */
+int qvd_batch_fixed_lanes(const qvd_batch *batch, uint32_t *out, uint32_t count);
 
 #ifdef __cplusplus
 }

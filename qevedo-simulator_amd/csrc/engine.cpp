@@ -20,6 +20,11 @@
 #include "vprog.h"
 #include "vstage.h"
 
+// Tiles per item of a batch's stage queue (at most).
+#ifndef QVD_QUEUE_CHUNK
+#define QVD_QUEUE_CHUNK 16
+#endif
+
 using namespace qvd;
 
 namespace qvd {
@@ -110,9 +115,13 @@ struct qvd_batch {
     uint64_t len = 0;            // amplitudes of all states
     uint64_t num_z = 0, num_zz = 0;
     uint64_t gates = 0, kernels = 0, stages = 0;
-    DBuf amps, ins, coefs, ldsops, ints, descs, block_start, basis;
+    DBuf amps, ins, coefs, ldsops, ints, descs, items, sync, basis;
     DBuf edescs, seg_start, out_start, seg_sums, z, zz;
-    uint32_t levels = 0;
+    uint32_t levels = 0, nitems = 0;
+    unsigned grid = 0;
+    std::vector<uint64_t> state_off;  // first amplitude of each circuit's state
+    std::vector<uint32_t> qubits;     // each circuit's qubit count
+    std::vector<uint32_t> fixed;      // each circuit's register-tile geometry (fixed lane bits)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~qvd_batch() {
@@ -780,7 +789,7 @@ std::vector<uint32_t> equivalent_variants(const std::vector<std::vector<Step>> &
 // QVD_PLANNER_LEARNED and a cost model installed, the variant whose stages
 // the model predicts to run fastest (the heuristic when no model is set).
 std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64_t num_ops, const qvd_options &opt,
-                            uint32_t *chosen = nullptr) {
+                            uint32_t *chosen = nullptr, uint32_t fixed_lanes = 0) {
     require(num_ops == 0 || ops != nullptr, "null ops");
     std::vector<Op> v;
     v.reserve(num_ops);
@@ -794,7 +803,8 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
     if (chosen) *chosen = 0;
     if (opt.tile_bits < 0) return plan_unblocked(v, opt.max_fused_qubits);
     TileConfig tc = default_tiles(precision);
-    const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    if (fixed_lanes) vg.f_bits = fixed_lanes;  // a batch's per-circuit geometry
     PlanParams p;
     p.num_qubits = n;
     p.max_fused = std::max<uint32_t>(opt.max_fused_qubits, 1);
@@ -935,13 +945,22 @@ void count_plan(const std::vector<Step> &steps, qvd_stats &st) {
     }
 }
 
+// Fails when a wait of the batch's stage queue ran out (after a replay).
+void check_queue(const qvd_batch *b) {
+    uint32_t err = 0;
+    QVD_HIP(hipMemcpy(&err, b->sync.as<uint32_t>() + VQUEUE_ERROR, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) fail(QVD_EHIP, "a batch stage waited more than 10 s for its previous stage; results are invalid");
+}
+
 // Builds a batch: plans each circuit on max(n, L) qubits (L = the register
 // tile), lowers its stages, lays out the level launch tables and the
 // expectation descriptors, uploads them and captures the graph.
 template <typename T2>
-void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, const qvd_options &opt_in) {
+void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, const qvd_options &opt_in,
+                 const std::vector<uint32_t> &fixed) {
     const VGeom vg = vstage_geom<T2>();
     const uint32_t L = vg.L();
+    b->fixed = fixed;
     qvd_options opt = opt_in;
     opt.tile_bits = 0;  // register tiles
     std::vector<uint64_t> state_off(count);
@@ -967,14 +986,16 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
                 require(o.controls && o.controls[j] < ci.num_qubits, "circuit " + std::to_string(c) + ": qubit out of range");
         }
         const uint32_t npad = std::max(ci.num_qubits, L);
-        std::vector<Step> steps = make_plan(npad, b->precision, ci.ops, ci.num_ops, opt);
+        VGeom gc = vg;
+        gc.f_bits = fixed[c];
+        std::vector<Step> steps = make_plan(npad, b->precision, ci.ops, ci.num_ops, opt, nullptr, gc.f_bits);
         PendingDiag pend(npad);
         state_off[c] = off;
         for (const Step &st : steps) {
             if (!st.is_stage || !st.stage.reg_tile)
                 fail(QVD_EUNSUPPORTED, "circuit " + std::to_string(c) +
                                            " needs a full-state pass; run it with qvd_sv_apply_ops");
-            VProgram p = build_vprogram(st.stage, npad, vg, pend, vco, vlds, vints, true);
+            VProgram p = build_vprogram(st.stage, npad, gc, pend, vco, vlds, vints, true);
             require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
             p.desc.instr_off = (uint32_t)vins.size();
             p.desc.state_off = off;
@@ -997,36 +1018,57 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
         off += uint64_t(1) << npad;
     }
     b->len = off;
-    // Level r: stage r of every circuit that has one; one workgroup per tile.
+    b->state_off = state_off;
+    for (uint32_t c = 0; c < count; c++) b->qubits.push_back(circuits[c].num_qubits);
+    // The stage queue: stage r of every circuit (level r) before stage r + 1
+    // of any, circuits in the same order on every level, so the tiles an item
+    // waits for were queued a whole level earlier. An item holds up to
+    // QVD_QUEUE_CHUNK tiles (fewer for states of few tiles, so that their
+    // stages still spread over several workgroups, and near the end of the
+    // queue), interleaved: item j of a
+    // stage of m items takes tiles j, j + m, j + 2m, ..., so the workgroups
+    // working on a stage at one time touch neighbouring tiles, as they do
+    // when each takes one tile. Its state's counter must first reach the
+    // tiles of the state's earlier stages.
     std::vector<VStageDesc> all_descs;
-    std::vector<uint32_t> all_bs;
-    struct Level {
-        uint32_t d0, nd, bs0;
-        unsigned grid;
-    };
-    std::vector<Level> levels;
+    std::vector<VQueueItem> items;
+    std::vector<uint64_t> queued(count, 0);
+    bool any_empty = false;
+    uint64_t remaining = 0;  // tiles of the queue from the current stage on
+    for (uint32_t c = 0; c < count; c++) {
+        any_empty |= stage_descs[c].empty();
+        for (const VStageDesc &d : stage_descs[c]) remaining += d.num_tiles;
+    }
+    unsigned resident = 0;
+    QVD_HIP(vstage_resident_groups<T2>(&resident));
     for (uint32_t r = 0;; r++) {
-        Level rd{(uint32_t)all_descs.size(), 0, (uint32_t)all_bs.size(), 0};
-        uint64_t blocks = 0;
-        all_bs.push_back(0);
+        bool any = false;
         for (uint32_t c = 0; c < count; c++) {
             if (r >= stage_descs[c].size()) continue;
-            const VStageDesc &d = stage_descs[c][r];
+            any = true;
+            VStageDesc d = stage_descs[c][r];
+            d.from_zero = r == 0;
+            const uint32_t di = (uint32_t)all_descs.size();
             all_descs.push_back(d);
-            blocks += d.num_tiles;
-            require(blocks < (uint64_t(1) << 31), "batch too large for one launch");
-            all_bs.push_back((uint32_t)blocks);
-            rd.nd++;
+            // Large items while much work remains (each item costs a fetch, a
+            // wait and a release), small ones towards the end of the queue,
+            // so that the workgroups finish together.
+            const uint64_t chunk = std::max<uint64_t>(
+                1, std::min<uint64_t>({QVD_QUEUE_CHUNK, d.num_tiles / 8, remaining / (4 * (uint64_t)resident)}));
+            remaining -= d.num_tiles;
+            const uint64_t m = (d.num_tiles + chunk - 1) / chunk;  // items of this stage
+            for (uint64_t j = 0; j < m; j++)
+                items.push_back(VQueueItem{di, c, (uint32_t)queued[c], (uint32_t)((d.num_tiles - j + m - 1) / m), j, m});
+            queued[c] += d.num_tiles;
+            require(queued[c] < (uint64_t(1) << 32), "circuit " + std::to_string(c) + " has too many stage tiles");
+            b->stages++;
         }
-        if (rd.nd == 0) {
-            all_bs.pop_back();
-            break;
-        }
-        rd.grid = (unsigned)blocks;
-        levels.push_back(rd);
-        b->stages += rd.nd;
+        if (!any) break;
+        b->levels++;
     }
-    b->levels = (uint32_t)levels.size();
+    require(items.size() < (uint64_t(1) << 32), "batch too large for one stage queue");
+    b->nitems = (uint32_t)items.size();
+    b->grid = (unsigned)std::min<uint64_t>(resident, std::max<uint64_t>(items.size(), 1));
     // Upload.
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
@@ -1041,7 +1083,8 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     upload(b->ldsops, vlds.data(), vlds.size() * sizeof(StageOp));
     upload(b->ints, vints.data(), vints.size() * sizeof(uint32_t));
     upload(b->descs, all_descs.data(), all_descs.size() * sizeof(VStageDesc));
-    upload(b->block_start, all_bs.data(), all_bs.size() * sizeof(uint32_t));
+    upload(b->items, items.data(), items.size() * sizeof(VQueueItem));
+    b->sync.alloc((VQUEUE_COUNTERS + count) * sizeof(uint32_t));
     upload(b->basis, state_off.data(), state_off.size() * sizeof(uint64_t));
     upload(b->edescs, ed.data(), ed.size() * sizeof(ExpDesc));
     upload(b->seg_start, seg_start.data(), seg_start.size() * sizeof(uint64_t));
@@ -1057,12 +1100,15 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     // capture), then the capture of the same sequence.
     T2 *s = b->amps.as<T2>();
     auto enqueue = [&]() {
-        launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
+        // Every circuit's first stage starts from |0...0> without loading; a
+        // circuit with no stage gets its state from the init.
+        if (any_empty) launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
+        launch_zero_words(b->sync.as<uint32_t>(), VQUEUE_COUNTERS + count, b->stream);
         QVD_HIP(hipGetLastError());
-        for (const Level &rd : levels)
-            QVD_HIP(launch_vstage<T2>(s, b->descs.as<VStageDesc>() + rd.d0, rd.nd, b->block_start.as<uint32_t>() + rd.bs0,
-                                      b->ins.as<VInstr>(), b->coefs.as<T2>(), b->ldsops.as<StageOp>(),
-                                      b->ints.as<uint32_t>(), rd.grid, b->stream));
+        if (b->nitems)
+            QVD_HIP(launch_vstage_queue<T2>(s, b->descs.as<VStageDesc>(), b->items.as<VQueueItem>(), b->nitems,
+                                            b->sync.as<uint32_t>(), b->ins.as<VInstr>(), b->coefs.as<T2>(),
+                                            b->ldsops.as<StageOp>(), b->ints.as<uint32_t>(), b->grid, b->stream));
         launch_expect_batch<T2>(s, b->edescs.as<ExpDesc>(), count, b->seg_start.as<uint64_t>(), seg_start[count],
                                 b->out_start.as<uint32_t>(), out_start[count], b->seg_sums.as<double>(),
                                 b->z.as<double>(), b->zz.as<double>(), b->stream);
@@ -1070,6 +1116,7 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     };
     enqueue();
     QVD_HIP(hipStreamSynchronize(b->stream));
+    check_queue(b);
     QVD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     try {
         enqueue();
@@ -1081,6 +1128,141 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     }
     QVD_HIP(hipStreamEndCapture(b->stream, &b->graph));
     QVD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+// A batch handle with its own stream on the current device.
+std::unique_ptr<qvd_batch> new_batch(int precision, uint32_t count) {
+    std::unique_ptr<qvd_batch> b(new qvd_batch());
+    b->precision = precision;
+    b->count = count;
+    QVD_HIP(hipGetDevice(&b->device));
+    QVD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    return b;
+}
+
+// The autotuner's choices for this process: class key -> fixed lane bits.
+struct TuneCache {
+    std::mutex mu;
+    std::unordered_map<uint64_t, uint32_t> choice;
+} g_tune;
+
+// A circuit's class for the autotuner: precision, size, and each op's
+// targets, controls and control values (not its matrix), FNV-1a.
+uint64_t circuit_class(const qvd_circuit &c, int precision, const qvd_options &opt) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        for (int i = 0; i < 8; i++) {
+            h ^= (v >> (8 * i)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    mix((uint64_t)precision);
+    mix(c.num_qubits);
+    mix(opt.max_fused_qubits);
+    mix(opt.variant);
+    mix(c.num_ops);
+    for (uint64_t i = 0; i < c.num_ops; i++) {
+        const qvd_op &o = c.ops[i];
+        mix(((uint64_t)o.num_targets << 32) | o.num_controls);
+        for (uint32_t j = 0; j < o.num_targets; j++) mix(o.targets[j]);
+        for (uint32_t j = 0; j < o.num_controls; j++) mix(((uint64_t)o.controls[j] << 1) | (o.control_values ? o.control_values[j] & 1 : 1));
+    }
+    return h;
+}
+
+// Seconds of the fastest of `reps` replays of a built batch's graph (after
+// one unmeasured replay).
+double replay_seconds(qvd_batch *b, int reps) {
+    QVD_HIP(hipGraphLaunch(b->exec, b->stream));
+    QVD_HIP(hipStreamSynchronize(b->stream));
+    double best = 1e30;
+    for (int r = 0; r < reps; r++) {
+        const double t0 = now();
+        QVD_HIP(hipGraphLaunch(b->exec, b->stream));
+        QVD_HIP(hipStreamSynchronize(b->stream));
+        best = std::min(best, now() - t0);
+    }
+    check_queue(b);
+    return best;
+}
+
+template <typename T2>
+void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, const qvd_options &opt_in,
+                 const std::vector<uint32_t> &fixed);
+
+// Each circuit's register-tile geometry: its own request, else the
+// autotuner's (QVD_FLAG_AUTOTUNE), else the default. The autotuner runs each
+// class not seen before as a batch of its own with every candidate geometry
+// (the default and one fixed lane bit fewer or more), on up to enough of the
+// class's circuits for 8 tiles per resident workgroup, and keeps the
+// fastest; a candidate must beat the default by 1 % to be kept.
+template <typename T2>
+std::vector<uint32_t> batch_geometry(const qvd_circuit *circuits, uint32_t count, int precision,
+                                     const qvd_options &opt) {
+    const VGeom vg = vstage_geom<T2>();
+    const uint32_t def = vg.f_bits, lo = def - 1, hi = def + 1;
+    std::vector<uint32_t> fixed(count, 0);
+    for (uint32_t c = 0; c < count; c++) {
+        const uint32_t f = circuits[c].fixed_lanes;
+        require(f == 0 || (f >= lo && f <= hi),
+                "circuit " + std::to_string(c) + ": fixed_lanes must be 0 or " + std::to_string(lo) + ".." + std::to_string(hi));
+        fixed[c] = f;
+    }
+    if (!(opt.flags & QVD_FLAG_AUTOTUNE)) {
+        for (uint32_t &f : fixed)
+            if (!f) f = def;
+        return fixed;
+    }
+    std::vector<uint64_t> keys(count, 0);
+    std::vector<uint64_t> order;  // class keys in order of first appearance
+    std::unordered_map<uint64_t, std::vector<uint32_t>> members;
+    for (uint32_t c = 0; c < count; c++) {
+        if (fixed[c]) continue;
+        keys[c] = circuit_class(circuits[c], precision, opt);
+        auto &m = members[keys[c]];
+        if (m.empty()) order.push_back(keys[c]);
+        m.push_back(c);
+    }
+    unsigned resident = 0;
+    QVD_HIP(vstage_resident_groups<T2>(&resident));
+    qvd_options popt = opt;
+    popt.flags &= ~QVD_FLAG_AUTOTUNE;
+    for (uint64_t key : order) {
+        {
+            std::lock_guard<std::mutex> lock(g_tune.mu);
+            auto hit = g_tune.choice.find(key);
+            if (hit != g_tune.choice.end()) continue;
+        }
+        // The probe: the class's first circuits, up to 8 tiles per resident
+        // workgroup per stage.
+        std::vector<qvd_circuit> probe;
+        uint64_t tiles = 0;
+        for (uint32_t c : members[key]) {
+            if (tiles >= 8ull * resident) break;
+            probe.push_back(circuits[c]);
+            tiles += uint64_t(1) << (std::max(circuits[c].num_qubits, vg.L()) - vg.L());
+        }
+        uint32_t best = def;
+        double best_s = 0, def_s = 0;
+        for (uint32_t f = lo; f <= hi; f++) {
+            std::unique_ptr<qvd_batch> pb = new_batch(precision, (uint32_t)probe.size());
+            build_batch<T2>(pb.get(), probe.data(), (uint32_t)probe.size(), popt,
+                            std::vector<uint32_t>(probe.size(), f));
+            const double t = replay_seconds(pb.get(), 3);
+            if (f == def) def_s = t;
+            if (f == lo || t < best_s) {
+                best_s = t;
+                best = f;
+            }
+        }
+        if (best != def && best_s > 0.99 * def_s) best = def;
+        std::lock_guard<std::mutex> lock(g_tune.mu);
+        g_tune.choice[key] = best;
+    }
+    std::lock_guard<std::mutex> lock(g_tune.mu);
+    for (uint32_t c = 0; c < count; c++)
+        if (!fixed[c]) fixed[c] = g_tune.choice.at(keys[c]);
+    return fixed;
 }
 
 // The sampler's buffers carved from one workspace (256-byte aligned parts);
@@ -1666,13 +1848,11 @@ int qvd_batch_create(const qvd_circuit *circuits, uint32_t count, int precision,
         if (options) opt = *options;
         int devices = 0;
         if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) fail(QVD_EHIP, "no HIP device is visible");
-        std::unique_ptr<qvd_batch> b(new qvd_batch());
-        b->precision = precision;
-        b->count = count;
-        QVD_HIP(hipGetDevice(&b->device));
-        QVD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        if (precision == QVD_C64) build_batch<float2>(b.get(), circuits, count, opt);
-        else build_batch<double2>(b.get(), circuits, count, opt);
+        std::vector<uint32_t> fixed = precision == QVD_C64 ? batch_geometry<float2>(circuits, count, precision, opt)
+                                                            : batch_geometry<double2>(circuits, count, precision, opt);
+        std::unique_ptr<qvd_batch> b = new_batch(precision, count);
+        if (precision == QVD_C64) build_batch<float2>(b.get(), circuits, count, opt, fixed);
+        else build_batch<double2>(b.get(), circuits, count, opt, fixed);
         *out = b.release();
     });
 }
@@ -1683,6 +1863,31 @@ int qvd_batch_destroy(qvd_batch *batch) {
         (void)hipSetDevice(batch->device);
         (void)hipStreamSynchronize(batch->stream);
         delete batch;
+    });
+}
+
+int qvd_batch_amplitudes(const qvd_batch *batch, uint32_t index, void *out, uint64_t capacity) {
+    return guard([&] {
+        require(batch != nullptr, "null batch handle");
+        require(index < batch->count, "circuit index out of range");
+        require(out != nullptr, "null output");
+        const uint64_t len = uint64_t(1) << batch->qubits[index];
+        require(capacity >= len, "output holds fewer than 2^num_qubits amplitudes");
+        const size_t amp = batch->precision == QVD_C64 ? sizeof(float2) : sizeof(double2);
+        QVD_HIP(hipSetDevice(batch->device));
+        // The ancillas are the state's top qubits, in |0>: the circuit's state
+        // is the first 2^n amplitudes.
+        QVD_HIP(hipMemcpy(out, batch->amps.as<char>() + batch->state_off[index] * amp, len * amp,
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int qvd_batch_fixed_lanes(const qvd_batch *batch, uint32_t *out, uint32_t count) {
+    return guard([&] {
+        require(batch != nullptr, "null batch handle");
+        require(out != nullptr || count == 0, "null output");
+        require(count <= batch->count, "count exceeds the batch's circuits");
+        for (uint32_t c = 0; c < count; c++) out[c] = batch->fixed[c];
     });
 }
 
@@ -1702,6 +1907,7 @@ int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats) {
         QVD_HIP(hipGraphLaunch(batch->exec, batch->stream));
         QVD_HIP(hipStreamSynchronize(batch->stream));
         const double t1 = now();
+        check_queue(batch);
         if (z && batch->num_z)
             QVD_HIP(hipMemcpy(z, batch->z.p, batch->num_z * sizeof(double), hipMemcpyDeviceToHost));
         if (zz && batch->num_zz)

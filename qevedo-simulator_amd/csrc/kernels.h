@@ -75,6 +75,8 @@ template <typename T2>
 void launch_expect_batch(const T2 *s, const ExpDesc *descs, uint32_t ndesc, const uint64_t *seg_start,
                          uint64_t total_segs, const uint32_t *out_start, uint32_t total_outs, double *seg_sums,
                          double *z, double *zz, hipStream_t st);
+// Zero-fills n 32-bit words with a kernel.
+void launch_zero_words(uint32_t *p, uint32_t n, hipStream_t st);
 // Zero-fills `len` amplitudes and sets amplitude offs[i] to 1 for each state i.
 template <typename T2>
 void launch_batch_init(T2 *s, uint64_t len, const uint64_t *offs, uint32_t count, hipStream_t st);

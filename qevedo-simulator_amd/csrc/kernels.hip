@@ -562,6 +562,13 @@ __global__ void k_set_basis(T2 *__restrict__ s, const uint64_t *__restrict__ off
     s[offs[i]] = v;
 }
 
+// Zero-fills n words (a kernel, not a memset: graph-captured memset nodes are
+// avoided).
+__global__ void k_zero_words(uint32_t *__restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Measurement collapse (src/measure.rs:68-80): the kept half scaled by
 // `keep` in the state's precision, the other half zeroed.
@@ -832,6 +839,10 @@ void launch_expect_batch(const T2 *s, const ExpDesc *descs, uint32_t ndesc, cons
     const ExpDesc none{};
     k_expect_segments<T2><<<(unsigned)total_segs, 256, 0, st>>>(s, none, descs, ndesc, seg_start, seg_sums);
     k_expect_combine<<<(total_outs + 63) / 64, 64, 0, st>>>(seg_sums, none, descs, ndesc, out_start, z, zz);
+}
+
+void launch_zero_words(uint32_t *p, uint32_t n, hipStream_t st) {
+    k_zero_words<<<(n + 255) / 256, 256, 0, st>>>(p, n);
 }
 
 template <typename T2>

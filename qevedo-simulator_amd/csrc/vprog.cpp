@@ -663,6 +663,7 @@ VProgram build_vprogram(const Stage &stage, uint32_t n, const VGeom &g, PendingD
     d.cbegin = cbegin;
     d.ccount = (uint32_t)coefs.size() - cbegin;
     d.L = (uint32_t)stage.local.size();
+    d.fixed_lanes = g.f_bits;
     for (uint32_t j = 0; j < d.L; j++) d.local_pos[j] = stage.local[j];
     d.nglob = 0;
     for (uint32_t q = 0; q < n; q++)

@@ -155,6 +155,8 @@ struct VStageDesc {
     uint8_t wmapF[VMAX_W_BITS];
     uint8_t rmapF[VMAX_R_BITS];
     uint32_t cbegin, ccount;         // the stage's coefficients, copied to LDS once per workgroup
+    uint32_t from_zero;              // nonzero: the state is |0...0> before this stage (its tiles are not loaded)
+    uint32_t fixed_lanes;            // F of the geometry the program was built for
 };
 
 // LDS budget of a stage's coefficient block, in bytes.

@@ -611,17 +611,23 @@ __device__ __forceinline__ void inline_coefs(const QVD_CONST VInstr &v, T2 (&c)[
     for (int j = 0; j < 4; j++) c[j] = p[j];
 }
 
-}  // namespace
-
-template <typename T2, int WB, int RB, int FB>
-__global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc *__restrict__ descs,
-                                                     uint32_t ndesc, const uint32_t *__restrict__ block_start,
-                                                     const VInstr *__restrict__ prog, const T2 *__restrict__ coefs,
-                                                     const StageOp *__restrict__ ldsops,
-                                                     const uint32_t *__restrict__ ints) {
+// Runs stage d on its tiles t_begin, t_begin + step, ... below t_end; s is
+// the first amplitude of the stage's state. With MAY_ZERO the stage may be
+// marked from_zero: its tiles are then made in registers (|0...0>), not
+// loaded. FBC is the number of fixed lane bits, or -1 to read it from the
+// descriptor (it enters only the offsets of the role maps, not the gate
+// handlers).
+template <typename T2, int WB, int RB, int FBC, bool MAY_ZERO>
+__device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST VStageDesc &d, uint64_t t_begin,
+                                            uint64_t t_end, uint64_t step, const VInstr *__restrict__ prog,
+                                            const T2 *__restrict__ coefs, const StageOp *__restrict__ ldsops,
+                                            const uint32_t *__restrict__ ints) {
     constexpr int W = 64 << WB, R = 1 << RB;
     constexpr uint32_t L = 6 + WB + RB;
-    extern __shared__ unsigned char smem_raw[];
+    // 16-byte aligned: dynamic LDS follows a kernel's static LDS (the queue
+    // kernel's item slot), and a tile at a 4-byte offset would split every
+    // 8- and 16-byte LDS access of the exchanges.
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T2 *tile = reinterpret_cast<T2 *>(smem_raw);
     T2 *cst = tile + (1u << L);  // the stage's coefficients (index = coef - cbegin)
     const QVD_CONST VInstr *cprog = (const QVD_CONST VInstr *)prog;
@@ -630,36 +636,21 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     const QVD_CONST uint32_t *cints = (const QVD_CONST uint32_t *)ints;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
 
-    // The workgroup's stage: descriptor di owns workgroups
-    // block_start[di] .. block_start[di + 1] - 1 (one descriptor: all).
-    uint32_t di = 0, bl = blockIdx.x, nb = gridDim.x;
-    if (ndesc > 1) {
-        const QVD_CONST uint32_t *cbs = (const QVD_CONST uint32_t *)block_start;
-        uint32_t lo = 0, hi = ndesc;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (cbs[mid] <= blockIdx.x) lo = mid;
-            else hi = mid;
-        }
-        di = lo;
-        bl = blockIdx.x - cbs[di];
-        nb = cbs[di + 1] - cbs[di];
-    }
-    const QVD_CONST VStageDesc &d = ((const QVD_CONST VStageDesc *)descs)[di];
-    s += d.state_off;
-
     for (uint32_t i = tid; i < d.ccount; i += W) cst[i] = coefs[d.cbegin + i];
     const T2 *cbase = cst - d.cbegin;  // so cbase[v.coef + i] is the LDS copy
     __syncthreads();
 
     // Physical offsets of the initial role map (loads) and the final one
     // (stores): thread part per lane, register part per register bit.
-    constexpr uint32_t FL = 6 - FB;  // flexible lane bits
+    const uint32_t FB = FBC >= 0 ? (uint32_t)FBC : d.fixed_lanes;
+    const uint32_t FL = 6 - FB;  // flexible lane bits
     uint64_t t_off0 = lane & ((1u << FB) - 1), t_offF = t_off0;
 #pragma unroll
-    for (int s_ = 0; s_ < (int)FL; s_++) {
-        t_off0 |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[cbyte(d.lmap0, s_)];
-        t_offF |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[cbyte(d.lmapF, s_)];
+    for (int s_ = 0; s_ < (int)VMAX_L_FLEX; s_++) {
+        if (s_ < (int)FL) {
+            t_off0 |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[cbyte(d.lmap0, s_)];
+            t_offF |= (uint64_t)((lane >> (FB + s_)) & 1) << d.local_pos[cbyte(d.lmapF, s_)];
+        }
     }
 #pragma unroll
     for (int s_ = 0; s_ < WB; s_++) {
@@ -679,7 +670,8 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     auto thread_index = [&](const QVD_CONST uint8_t *lm, const QVD_CONST uint8_t *wmp) {
         uint32_t ti = lane & ((1u << FB) - 1);
 #pragma unroll
-        for (int s_ = 0; s_ < (int)FL; s_++) ti |= ((lane >> (FB + s_)) & 1) << cbyte(lm, s_);
+        for (int s_ = 0; s_ < (int)VMAX_L_FLEX; s_++)
+            if (s_ < (int)FL) ti |= ((lane >> (FB + s_)) & 1) << cbyte(lm, s_);
 #pragma unroll
         for (int s_ = 0; s_ < WB; s_++) ti |= ((tid >> (6 + s_)) & 1) << cbyte(wmp, s_);
         return ti;
@@ -721,14 +713,24 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
     // (the memory counter is shared by loads and stores, and a pending load
     // at the loop head would make every tile wait for its predecessor's
     // stores).
-    uint64_t t = bl;
-    if (t < d.num_tiles) load(x, tile_base(t));
+    // A stage that starts from |0...0>: amplitude 0 (tile 0, thread offset 0,
+    // register 0) is 1, every other one 0.
+    const bool zero = MAY_ZERO && d.from_zero != 0;
+    auto make_zero = [&](T2(&dst)[R], uint64_t t) {
+#pragma unroll
+        for (int j = 0; j < R; j++) dst[j] = mk<T2>(j == 0 && t == 0 && t_off0 == 0 ? 1 : 0, 0);
+    };
+    uint64_t t = t_begin;
+    if (t < t_end) {
+        if (zero) make_zero(x, t);
+        else load(x, tile_base(t));
+    }
     if constexpr (PREFETCH) __builtin_amdgcn_s_waitcnt(0);
-    for (; t < d.num_tiles; t += nb) {
+    for (; t < t_end; t += step) {
         const uint64_t base = tile_base(t);
-        const uint64_t tn = t + nb;
+        const uint64_t tn = t + step;
         if constexpr (PREFETCH)
-            if (tn < d.num_tiles) load(y, tile_base(tn));
+            if (tn < t_end && !zero) load(y, tile_base(tn));
 
         // Current role map (an exchange changes it; each tile starts from
         // the stage's initial one).
@@ -890,11 +892,122 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         }
         // The next tile's first exchange writes LDS only after a barrier, so
         // no barrier is needed here.
-        if constexpr (PREFETCH) {
+        if (zero) {
+            if (tn < t_end) make_zero(x, tn);
+        } else if constexpr (PREFETCH) {
 #pragma unroll
             for (int j = 0; j < R; j++) x[j] = y[j];
-        } else if (tn < d.num_tiles) {
+        } else if (tn < t_end) {
             load(x, tile_base(tn));
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T2, int WB, int RB, int FB>
+__global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc *__restrict__ descs,
+                                                     uint32_t ndesc, const uint32_t *__restrict__ block_start,
+                                                     const VInstr *__restrict__ prog, const T2 *__restrict__ coefs,
+                                                     const StageOp *__restrict__ ldsops,
+                                                     const uint32_t *__restrict__ ints) {
+    // The workgroup's stage: descriptor di owns workgroups
+    // block_start[di] .. block_start[di + 1] - 1 (one descriptor: all).
+    uint32_t di = 0, bl = blockIdx.x, nb = gridDim.x;
+    if (ndesc > 1) {
+        const QVD_CONST uint32_t *cbs = (const QVD_CONST uint32_t *)block_start;
+        uint32_t lo = 0, hi = ndesc;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cbs[mid] <= blockIdx.x) lo = mid;
+            else hi = mid;
+        }
+        di = lo;
+        bl = blockIdx.x - cbs[di];
+        nb = cbs[di + 1] - cbs[di];
+    }
+    const QVD_CONST VStageDesc &d = ((const QVD_CONST VStageDesc *)descs)[di];
+    stage_tiles<T2, WB, RB, FB, false>(s + d.state_off, d, bl, d.num_tiles, nb, prog, coefs, ldsops, ints);
+}
+
+// Every stage of a batch of states in one launch. Items (runs of tiles of one
+// stage) are taken in array order from a counter; an item's tiles read what
+// every tile of its state's previous stage wrote, so the item waits until the
+// state's counter of finished tiles has reached item.need. An item's
+// dependencies all come earlier in the array and a workgroup only waits
+// after it has started, so the earliest unfinished item taken always has its
+// inputs and the queue drains with any number of resident workgroups.
+//
+// Tiles of one state are written and read on different XCDs, whose L2
+// caches are not coherent with each other: every thread waits for its
+// stores to reach L2 before the workgroup's count is released at agent scope
+// (which writes L2 back), and a wait ends with an agent-scope acquire (which
+// invalidates the stale lines) before any load of the item.
+template <typename T2, int WB, int RB, int FB>
+__global__ __launch_bounds__(64 << WB) void k_vstage_queue(T2 *__restrict__ s, const VStageDesc *__restrict__ descs,
+                                                           const VQueueItem *__restrict__ items, uint32_t nitems,
+                                                           uint32_t *__restrict__ sync,
+                                                           const VInstr *__restrict__ prog,
+                                                           const T2 *__restrict__ coefs,
+                                                           const StageOp *__restrict__ ldsops,
+                                                           const uint32_t *__restrict__ ints) {
+    __shared__ uint32_t slot;
+    uint32_t *const head = sync + VQUEUE_HEAD, *const error = sync + VQUEUE_ERROR;
+    uint32_t *const done = sync + VQUEUE_COUNTERS;
+    const QVD_CONST VQueueItem *citems = (const QVD_CONST VQueueItem *)items;
+    // Wave 0 takes the items and waits. Every branch of this loop is wave-
+    // uniform (scalar conditions; all 64 lanes of wave 0 run its atomics,
+    // lane 0 adding the amount and the others 0): a branch on one lane
+    // around the barriers lets the compiler rotate the loop so that the other
+    // lanes of wave 0 pass the next barrier before lane 0 has taken an item.
+    // Taking the next item before finishing this one keeps the queue
+    // deadlock-free: the earliest unfinished item is always one whose
+    // workgroup is running it or waiting only on finished items.
+    const bool lead = __builtin_amdgcn_readfirstlane(threadIdx.x) == 0;
+    const uint32_t first = (threadIdx.x & 63) == 0;
+    // The first item is taken here, every later one while the previous
+    // item's stores drain.
+    uint32_t next = 0;
+    if (lead)
+        next = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_fetch_add(head, first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (;;) {
+        if (lead) {
+            const uint32_t i = next;
+            if (i < nitems && citems[i].need != 0) {
+                const uint32_t st = citems[i].state, need = citems[i].need;
+                const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+                for (;;) {
+                    const uint32_t have = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(done + st, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                    if (have >= need) break;
+                    if (__builtin_amdgcn_readfirstlane(
+                            __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+                        break;
+                    if (__builtin_amdgcn_s_memrealtime() - t0 > VQUEUE_WAIT_TICKS) {
+                        __hip_atomic_store(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(8);
+                }
+            }
+            slot = i;
+        }
+        __syncthreads();
+        const uint32_t i = __builtin_amdgcn_readfirstlane(slot);
+        if (i >= nitems) break;
+        const QVD_CONST VQueueItem &it = citems[i];
+        const QVD_CONST VStageDesc &d = ((const QVD_CONST VStageDesc *)descs)[it.desc];
+        stage_tiles<T2, WB, RB, -1, true>(s + d.state_off, d, it.tile0, it.tile0 + it.count * it.step, it.step, prog,
+                                          coefs, ldsops, ints);
+        uint32_t got = 0;
+        if (lead) got = __hip_atomic_fetch_add(head, first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's stores (and wave 0's fetch) have completed
+        __syncthreads();
+        if (lead) {
+            __hip_atomic_fetch_add(done + it.state, first ? it.count : 0u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            next = __builtin_amdgcn_readfirstlane(got);
         }
     }
 }
@@ -936,6 +1049,59 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
     return hipGetLastError();
 }
 
+namespace {
+
+template <typename T2>
+auto vstage_queue_kernel() {
+    if constexpr (sizeof(T2) == 8) return k_vstage_queue<T2, 4, 4, 4>;
+    else return k_vstage_queue<T2, 3, 4, 3>;
+}
+
+// Sets the queue kernel's dynamic LDS limit once per device (outside any
+// graph capture: the first call comes from an eager run).
+template <typename T2>
+hipError_t queue_attributes() {
+    static std::atomic<uint64_t> attr_set{0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = uint64_t(1) << (dev & 63);
+    if (!(attr_set.load() & bit)) {
+        e = hipFuncSetAttribute((const void *)vstage_queue_kernel<T2>(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vstage_lds_bytes<T2>());
+        if (e != hipSuccess) return e;
+        attr_set.fetch_or(bit);
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+template <typename T2>
+hipError_t vstage_resident_groups(unsigned *groups) {
+    hipError_t e = queue_attributes<T2>();
+    if (e != hipSuccess) return e;
+    int dev = 0, per_cu = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)vstage_queue_kernel<T2>(),
+                                                     64 << vstage_geom<T2>().w_bits, vstage_lds_bytes<T2>());
+    if (e != hipSuccess) return e;
+    *groups = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus);
+    return hipSuccess;
+}
+
+template <typename T2>
+hipError_t launch_vstage_queue(T2 *s, const VStageDesc *descs, const VQueueItem *items, uint32_t nitems,
+                               uint32_t *sync, const VInstr *prog, const T2 *coefs, const StageOp *ldsops,
+                               const uint32_t *ints, unsigned grid, hipStream_t st) {
+    hipError_t e = queue_attributes<T2>();
+    if (e != hipSuccess) return e;
+    vstage_queue_kernel<T2>()<<<grid, 64 << vstage_geom<T2>().w_bits, vstage_lds_bytes<T2>(), st>>>(
+        s, descs, items, nitems, sync, prog, coefs, ldsops, ints);
+    return hipGetLastError();
+}
+
 template VGeom vstage_geom<float2>();
 template VGeom vstage_geom<double2>();
 template size_t vstage_lds_bytes<float2>();
@@ -944,5 +1110,13 @@ template hipError_t launch_vstage<float2>(float2 *, const VStageDesc *, uint32_t
                                           const float2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t);
 template hipError_t launch_vstage<double2>(double2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
                                            const double2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t);
+template hipError_t launch_vstage_queue<float2>(float2 *, const VStageDesc *, const VQueueItem *, uint32_t, uint32_t *,
+                                                const VInstr *, const float2 *, const StageOp *, const uint32_t *,
+                                                unsigned, hipStream_t);
+template hipError_t launch_vstage_queue<double2>(double2 *, const VStageDesc *, const VQueueItem *, uint32_t,
+                                                 uint32_t *, const VInstr *, const double2 *, const StageOp *,
+                                                 const uint32_t *, unsigned, hipStream_t);
+template hipError_t vstage_resident_groups<float2>(unsigned *);
+template hipError_t vstage_resident_groups<double2>(unsigned *);
 
 }  // namespace qvd

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("QVD_LIB", os.path.join(_PKG_ROOT, "lib", "libqvd_hip.
 QVD_OK, QVD_EINVAL, QVD_ENOMEM, QVD_EHIP, QVD_EUNSUPPORTED = 0, -1, -2, -3, -4
 C64, C128 = 0, 1
 PLANNER_HEURISTIC, PLANNER_LEARNED, PLANNER_LEARNED_ANY = 0, 1, 2
-FLAG_TIMING, FLAG_DENSE_VALU = 1, 2
+FLAG_TIMING, FLAG_DENSE_VALU, FLAG_AUTOTUNE = 1, 2, 4
 
 # Every symbol include/qvd_hip.h declares.
 EXPORTS = (
@@ -27,7 +27,7 @@ EXPORTS = (
     "qvd_sv_apply_ops", "qvd_plan_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
     "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_compare", "qvd_sv_last_sample_seconds",
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
-    "qvd_batch_outputs", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
+    "qvd_batch_outputs", "qvd_batch_amplitudes", "qvd_batch_fixed_lanes", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
     "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_sample_noalloc", "qvd_sv_expect_pauli", "qvd_sv_copy",
 )
@@ -66,6 +66,7 @@ class CircuitDesc(C.Structure):
         ("num_qubits", C.c_uint32),
         ("ops", C.POINTER(Op)),
         ("num_ops", C.c_uint64),
+        ("fixed_lanes", C.c_uint32),
     ]
 
 
@@ -131,6 +132,8 @@ def lib():
         L.qvd_batch_destroy.argtypes = [vp]
         L.qvd_batch_run.argtypes = [vp, P(d), P(d), P(Stats)]
         L.qvd_batch_outputs.argtypes = [vp, P(u64), P(u64)]
+        L.qvd_batch_amplitudes.argtypes = [vp, C.c_uint32, vp, u64]
+        L.qvd_batch_fixed_lanes.argtypes = [vp, P(u32), u32]
         f = C.c_float
         L.qvd_plan_features.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(f), u64, P(u64)]
         L.qvd_plan_equivalent_variants.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(u32)]
@@ -450,18 +453,22 @@ class StateVector:
 class Batch:
     """Many circuits, each on its own state, run together (qvd_batch_*).
 
-    circuits: a list of (num_qubits, lowered ops). run() returns, per circuit,
-    (<Z_q> for every qubit, <Z_q Z_{q+1}> for neighbouring qubits).
+    circuits: a list of (num_qubits, lowered ops) or (num_qubits, lowered ops,
+    fixed_lanes), fixed_lanes being the register-tile geometry to build the
+    circuit with (0 = default). autotune: time the candidate geometries per
+    class of circuits (qvd_batch_create with QVD_FLAG_AUTOTUNE). run()
+    returns, per circuit, (<Z_q> for every qubit, <Z_q Z_{q+1}> for
+    neighbouring qubits).
     """
 
-    def __init__(self, circuits, precision: str = "single", max_fused_qubits: int = 4):
+    def __init__(self, circuits, precision: str = "single", max_fused_qubits: int = 4, autotune: bool = False):
         self.h = None
-        self.sizes = [int(n) for n, _ in circuits]
-        self.arrays = [ops if isinstance(ops, OpArray) else OpArray(ops) for _, ops in circuits]
+        self.sizes = [int(c[0]) for c in circuits]
+        self.arrays = [c[1] if isinstance(c[1], OpArray) else OpArray(c[1]) for c in circuits]
         cs = (CircuitDesc * len(self.sizes))()
-        for i, (n, arr) in enumerate(zip(self.sizes, self.arrays)):
-            cs[i] = CircuitDesc(n, arr.arr, arr.n)
-        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0, 0)
+        for i, (c, arr) in enumerate(zip(circuits, self.arrays)):
+            cs[i] = CircuitDesc(int(c[0]), arr.arr, arr.n, int(c[2]) if len(c) > 2 else 0)
+        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, FLAG_AUTOTUNE if autotune else 0, 0)
         h = C.c_void_p()
         _check(lib().qvd_batch_create(cs, len(self.sizes), {"single": C64, "double": C128}[precision], C.byref(opt),
                                       C.byref(h)))
@@ -470,6 +477,19 @@ class Batch:
         _check(lib().qvd_batch_outputs(self.h, C.byref(nz), C.byref(nzz)))
         self.num_z, self.num_zz = nz.value, nzz.value
         self.stats = {}
+        self.precision = precision
+
+    def fixed_lanes(self) -> list:
+        """The register-tile geometry (fixed lane bits) of each circuit."""
+        out = (C.c_uint32 * len(self.sizes))()
+        _check(lib().qvd_batch_fixed_lanes(self.h, out, len(self.sizes)))
+        return list(out)
+
+    def amplitudes(self, index: int) -> np.ndarray:
+        """Circuit `index`'s state after the last run, in the batch's precision."""
+        out = np.empty(1 << self.sizes[index], dtype=np.complex64 if self.precision == "single" else np.complex128)
+        _check(lib().qvd_batch_amplitudes(self.h, index, out.ctypes.data_as(C.c_void_p), out.size))
+        return out
 
     def run(self) -> list:
         z = np.zeros(max(self.num_z, 1), dtype=np.float64)
