@@ -117,9 +117,11 @@ def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
     """Batching workload: `count` hardware-efficient-ansatz circuits (6 layers
     of RY, RZ on every qubit plus a CZ ladder; qubit counts uniform in 12..22,
     angles uniform in [0, 2 pi)) in complex64, <Z_i> and <Z_i Z_i+1> of each:
-    one graph-captured batch against the same circuits run one by one on the
-    GPU, and the first `cpu_sample` of them one by one on the oracle's CPU
-    path."""
+    one graph-captured batch (every stage of every circuit in one launch of
+    the stage-queue kernel, register-tile geometry autotuned per circuit
+    class) against the same batch with the default geometry, the same
+    circuits run one by one on the GPU, and the first `cpu_sample` of them
+    one by one on the oracle's CPU path."""
     import numpy as np
     from qevedo.simulator import Batch, StateVector, library, lower_gates
     from qevedo.simulator import _native
@@ -128,25 +130,30 @@ def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
     sizes = [int(x) for x in rng.integers(12, 23, size=count)]
     gate_lists = [library.hardware_efficient_ansatz(n, 6, 1000 + i) for i, n in enumerate(sizes)]
     circuits = [(n, _native.OpArray(lower_gates(g))) for n, g in zip(sizes, gate_lists)]
+    def timed(b, reps=5):
+        b.run()
+        t = time.perf_counter()
+        for _ in range(reps):
+            out = b.run()
+        return (time.perf_counter() - t) / reps, out
+
     t = time.perf_counter()
-    b = Batch(circuits, "single")
+    b = Batch(circuits, "single", autotune=True)
     build_s = time.perf_counter() - t
-    b.run()
-    reps = 5
-    t = time.perf_counter()
-    for _ in range(reps):
-        out = b.run()
-    batch_s = (time.perf_counter() - t) / reps
+    batch_s, out = timed(b)
     stats = b.stats
+    lanes, stage_counts = b.fixed_lanes(), b.stages()
     b.close()
-    # Algorithmic HBM bytes of one replay: every stage of every circuit reads
-    # and writes its state (widened to at least one 14-qubit tile), the reset
-    # writes it and the expectation values read it.
-    passes_bytes = 0
-    for n, ops in circuits:
-        npad = max(n, 14)
-        stages = _native.plan_ops(npad, ops)["stages"]
-        passes_bytes += (2 * stages + 2) * 8 * (1 << npad)
+    with Batch(circuits, "single") as d:
+        default_s, _ = timed(d)
+        default_stages = sum(d.stages())
+    geometry = {str(n): sorted({f for f, m in zip(lanes, sizes) if m == n}) for n in sorted(set(sizes))}
+    # Algorithmic HBM bytes of one replay: each circuit's state (widened to
+    # at least one 14-qubit tile) is written by its first stage (made from
+    # |0...0> in registers, not read), read and written by every later stage
+    # and read by the expectation values: 2 x stages passes of 8 bytes per
+    # amplitude.
+    passes_bytes = sum(2 * k * 8 * (1 << max(n, 14)) for n, k in zip(sizes, stage_counts))
     achieved = passes_bytes / batch_s / 1e9
     # the same circuits one by one on the GPU: plan, run and expectation values per state
     t = time.perf_counter()
@@ -185,13 +192,16 @@ def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
         cpu = {"value": cpu_s * count / k, "unit": "s", "cores": threads, "kind": "port",
                "sample": f"the first {k} of the {count} circuits one by one (oracle apply + <Z>, <ZZ> from the "
                          f"probabilities), scaled to {count}", "max_abs_diff_vs_batch": cworst}
-    log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run ({stats['passes']} stage levels, "
-        f"{stats['stages']} stage descriptors; {achieved:.0f} GB/s algorithmic; build {build_s:.2f} s); "
+    log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run, one stage launch ({stats['stages']} "
+        f"stages, at most {stats['passes']} per circuit; {achieved:.0f} GB/s algorithmic; build and autotune "
+        f"{build_s:.2f} s; default geometry {default_s * 1e3:.2f} ms, {default_stages} stages); "
         f"one by one on the GPU {single_s:.3f} s; max |diff| {worst:.2e}"
         + (f"; CPU one by one {cpu['value']:.2f} s (sampled)" if cpu else ""))
     return {"circuits": count, "qubits": "12..22", "seconds": batch_s, "one_by_one_seconds": single_s,
-            "speedup": single_s / batch_s, "stage_levels": stats["passes"], "build_seconds": build_s,
-            "max_abs_diff_vs_one_by_one": worst,
+            "speedup": single_s / batch_s, "stage_launches_per_replay": 1, "stages": stats["stages"],
+            "max_stages_per_circuit": stats["passes"], "build_and_autotune_seconds": build_s,
+            "fixed_lanes_by_qubits": geometry, "default_geometry_seconds": default_s,
+            "default_geometry_stages": default_stages, "max_abs_diff_vs_one_by_one": worst,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_replay": passes_bytes},
             "cpu_baseline": cpu}

@@ -320,9 +320,10 @@ int qvd_batch_outputs(const qvd_batch *batch, uint64_t *num_z, uint64_t *num_zz)
  * need the states themselves, and for checking a batch against single
  * states amplitude by amplitude. */
 int qvd_batch_amplitudes(const qvd_batch *batch, uint32_t index, void *out, uint64_t capacity);
-/* The register-tile geometry (fixed lane bits) each circuit was built with,
- * `count` values into out. No reference counterpart. */
-int qvd_batch_fixed_lanes(const qvd_batch *batch, uint32_t *out, uint32_t count);
+/* The register-tile geometry (fixed lane bits) each of the first `count`
+ * circuits was built with, and its number of stages; either output may be
+ * NULL. No reference counterpart. */
+int qvd_batch_geometry(const qvd_batch *batch, uint32_t *fixed_lanes, uint32_t *stages, uint32_t count);
 
 #ifdef __cplusplus
 }

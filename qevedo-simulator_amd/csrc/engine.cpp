@@ -122,6 +122,7 @@ struct qvd_batch {
     std::vector<uint64_t> state_off;  // first amplitude of each circuit's state
     std::vector<uint32_t> qubits;     // each circuit's qubit count
     std::vector<uint32_t> fixed;      // each circuit's register-tile geometry (fixed lane bits)
+    std::vector<uint32_t> circuit_stages;  // each circuit's number of stages
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~qvd_batch() {
@@ -1019,7 +1020,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     }
     b->len = off;
     b->state_off = state_off;
-    for (uint32_t c = 0; c < count; c++) b->qubits.push_back(circuits[c].num_qubits);
+    for (uint32_t c = 0; c < count; c++) {
+        b->qubits.push_back(circuits[c].num_qubits);
+        b->circuit_stages.push_back((uint32_t)stage_descs[c].size());
+    }
     // The stage queue: stage r of every circuit (level r) before stage r + 1
     // of any, circuits in the same order on every level, so the tiles an item
     // waits for were queued a whole level earlier. An item holds up to
@@ -1882,12 +1886,14 @@ int qvd_batch_amplitudes(const qvd_batch *batch, uint32_t index, void *out, uint
     });
 }
 
-int qvd_batch_fixed_lanes(const qvd_batch *batch, uint32_t *out, uint32_t count) {
+int qvd_batch_geometry(const qvd_batch *batch, uint32_t *fixed_lanes, uint32_t *stages, uint32_t count) {
     return guard([&] {
         require(batch != nullptr, "null batch handle");
-        require(out != nullptr || count == 0, "null output");
         require(count <= batch->count, "count exceeds the batch's circuits");
-        for (uint32_t c = 0; c < count; c++) out[c] = batch->fixed[c];
+        for (uint32_t c = 0; c < count; c++) {
+            if (fixed_lanes) fixed_lanes[c] = batch->fixed[c];
+            if (stages) stages[c] = batch->circuit_stages[c];
+        }
     });
 }
 

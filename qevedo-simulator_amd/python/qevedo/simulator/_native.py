@@ -27,7 +27,7 @@ EXPORTS = (
     "qvd_sv_apply_ops", "qvd_plan_ops", "qvd_sv_norm_sqr", "qvd_sv_scale", "qvd_sv_probability_one", "qvd_sv_probabilities",
     "qvd_sv_expect_z", "qvd_sv_measure", "qvd_sv_reset", "qvd_sv_sample", "qvd_sv_compare", "qvd_sv_last_sample_seconds",
     "qvd_gate_matrix", "qvd_gate_lower", "qvd_batch_create", "qvd_batch_destroy", "qvd_batch_run",
-    "qvd_batch_outputs", "qvd_batch_amplitudes", "qvd_batch_fixed_lanes", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
+    "qvd_batch_outputs", "qvd_batch_amplitudes", "qvd_batch_geometry", "qvd_plan_features", "qvd_sv_stage_seconds", "qvd_mlp_create", "qvd_mlp_destroy",
     "qvd_mlp_num_params", "qvd_mlp_set_params", "qvd_mlp_get_params", "qvd_mlp_forward", "qvd_mlp_gradient",
     "qvd_mlp_train", "qvd_planner_set_model", "qvd_state_vector_fits", "qvd_plan_equivalent_variants", "qvd_sv_sample_noalloc", "qvd_sv_expect_pauli", "qvd_sv_copy",
 )
@@ -133,7 +133,7 @@ def lib():
         L.qvd_batch_run.argtypes = [vp, P(d), P(d), P(Stats)]
         L.qvd_batch_outputs.argtypes = [vp, P(u64), P(u64)]
         L.qvd_batch_amplitudes.argtypes = [vp, C.c_uint32, vp, u64]
-        L.qvd_batch_fixed_lanes.argtypes = [vp, P(u32), u32]
+        L.qvd_batch_geometry.argtypes = [vp, P(u32), P(u32), u32]
         f = C.c_float
         L.qvd_plan_features.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(f), u64, P(u64)]
         L.qvd_plan_equivalent_variants.argtypes = [u32, C.c_int, P(Op), u64, P(Options), P(u32)]
@@ -482,7 +482,13 @@ class Batch:
     def fixed_lanes(self) -> list:
         """The register-tile geometry (fixed lane bits) of each circuit."""
         out = (C.c_uint32 * len(self.sizes))()
-        _check(lib().qvd_batch_fixed_lanes(self.h, out, len(self.sizes)))
+        _check(lib().qvd_batch_geometry(self.h, out, None, len(self.sizes)))
+        return list(out)
+
+    def stages(self) -> list:
+        """The number of register-tile stages of each circuit."""
+        out = (C.c_uint32 * len(self.sizes))()
+        _check(lib().qvd_batch_geometry(self.h, None, out, len(self.sizes)))
         return list(out)
 
     def amplitudes(self, index: int) -> np.ndarray:
