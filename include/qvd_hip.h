@@ -126,12 +126,16 @@ int qvd_device_info(int *num_devices, char *arch, uint32_t arch_capacity);
 int qvd_state_vector_fits(uint32_t num_qubits, int precision, int *fits, uint64_t *state_bytes,
                           uint64_t *device_bytes);
 
-/* StateVector::zero (src/state.rs:39-49): |0...0> in HBM. */
+/* StateVector::zero (src/state.rs:39-49): |0...0> in HBM. The zeros are
+ * written lazily: the next qvd_sv_apply_ops whose plan starts with a
+ * register-tile stage makes that stage's tiles from |0...0> in registers (no
+ * init pass, no loads), and any other call writes them first. */
 int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out);
 int qvd_sv_destroy(qvd_sv *sv);
 int qvd_sv_num_qubits(const qvd_sv *sv, uint32_t *out);
 int qvd_sv_precision(const qvd_sv *sv, int *out);
-/* Reset to |0...0> without reallocating. */
+/* Reset to |0...0> without reallocating, written lazily as for
+ * qvd_sv_create. */
 int qvd_sv_set_zero(qvd_sv *sv);
 
 /* A copy of src's amplitudes in dst (same size and precision), device to

@@ -94,6 +94,7 @@ struct qvd_sv {
     DBuf plan_ws;    // a plan's uploaded programs, tables and matrices, grown to the largest plan and kept
     double last_sample_seconds = 0;
     std::vector<double> stage_seconds;  // register-tile stage kernels of the last timed apply_ops
+    bool zero_pending = false;          // the state is |0...0> but the amplitudes are not written yet
 
     size_t elem() const { return precision == QVD_C64 ? sizeof(float2) : sizeof(double2); }
     void bind() const { QVD_HIP(hipSetDevice(device)); }
@@ -146,7 +147,27 @@ auto with_type(const qvd_sv *sv, F &&f) {
     return f((double2 *)sv->amps.p);
 }
 
-void check_handle(const qvd_sv *sv) { require(sv != nullptr, "null state handle"); }
+// |0...0> is written lazily: qvd_sv_create and qvd_sv_set_zero only mark the
+// state, the next qvd_sv_apply_ops whose plan starts with a register-tile
+// stage makes that stage's tiles from |0...0> in registers (no init pass and
+// no tile loads), and any other use of the state writes the zeros first.
+void materialize(qvd_sv *sv) {
+    if (!sv->zero_pending) return;
+    sv->bind();
+    with_type(sv, [&](auto *s) {
+        launch_init_zero(s, sv->len, sv->stream);
+        return 0;
+    });
+    QVD_HIP(hipGetLastError());
+    sv->sync();
+    sv->zero_pending = false;
+}
+
+// Every entry point that uses a state's amplitudes starts here.
+void check_handle(const qvd_sv *sv) {
+    require(sv != nullptr, "null state handle");
+    materialize(const_cast<qvd_sv *>(sv));
+}
 
 // Validates qubits of an op the way the reference asserts them
 // (src/state.rs:182-188; python/src/lib.rs:29-43).
@@ -422,7 +443,8 @@ struct LaunchTimer {
 };
 
 template <typename T2>
-void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags) {
+void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags,
+              bool from_zero = false) {
     const bool timing = (flags & QVD_FLAG_TIMING) != 0;
     LaunchTimer timer;
     timer.on = timing;
@@ -446,6 +468,8 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             VProgram p = build_vprogram(step.stage, sv->n, geom, pend, vco, vlds, vints, true);
             require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
             p.desc.instr_off = (uint32_t)vins.size();
+            // The caller passes from_zero only when this is the plan's first step.
+            p.desc.from_zero = from_zero && vdescs.empty() && &step == &steps[0];
             vins.insert(vins.end(), p.instr.begin(), p.instr.end());
             vdescs.push_back(p.desc);
             continue;
@@ -485,7 +509,8 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             const unsigned grid = (unsigned)std::min<uint64_t>(d.num_tiles, (uint64_t)cus);
             timer.begin(timer.stage);
             reg_launch.push_back(1);
-            QVD_HIP(launch_vstage<T2>(s, dvdescs + vi, 1, nullptr, dvins, dvco, dvlds, dvints, grid, sv->stream));
+            QVD_HIP(launch_vstage<T2>(s, dvdescs + vi, 1, nullptr, dvins, dvco, dvlds, dvints, grid, sv->stream,
+                                      d.from_zero != 0));
             vi++;
             timer.end(timer.stage);
             st.stages++;
@@ -1383,12 +1408,7 @@ int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out) {
         sv->bind();
         QVD_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
         sv->amps.alloc(bytes);
-        with_type(sv.get(), [&](auto *s) {
-            launch_init_zero(s, sv->len, sv->stream);
-            return 0;
-        });
-        QVD_HIP(hipGetLastError());
-        sv->sync();
+        sv->zero_pending = true;
         *out = sv.release();
     });
 }
@@ -1421,24 +1441,20 @@ int qvd_sv_precision(const qvd_sv *sv, int *out) {
 
 int qvd_sv_set_zero(qvd_sv *sv) {
     return guard([&] {
-        check_handle(sv);
-        sv->bind();
-        with_type(sv, [&](auto *s) {
-            launch_init_zero(s, sv->len, sv->stream);
-            return 0;
-        });
-        QVD_HIP(hipGetLastError());
-        sv->sync();
+        require(sv != nullptr, "null state handle");
+        sv->zero_pending = true;
     });
 }
 
 int qvd_sv_copy(qvd_sv *dst, const qvd_sv *src) {
     return guard([&] {
-        check_handle(dst);
-        check_handle(src);
+        require(dst != nullptr && src != nullptr, "null state handle");
         require(dst->n == src->n && dst->precision == src->precision, "states of different sizes or precisions");
         require(dst->device == src->device, "states on different devices");
         if (dst == src) return;
+        // dst is overwritten whole: a pending |0...0> in src is copied as pending.
+        dst->zero_pending = src->zero_pending;
+        if (src->zero_pending) return;
         src->sync();
         dst->bind();
         QVD_HIP(hipMemcpyAsync(dst->amps.p, src->amps.p, src->len * src->elem(), hipMemcpyDeviceToDevice,
@@ -1449,9 +1465,10 @@ int qvd_sv_copy(qvd_sv *dst, const qvd_sv *src) {
 
 int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len) {
     return guard([&] {
-        check_handle(sv);
+        require(sv != nullptr, "null state handle");
         require(amplitudes != nullptr, "null amplitudes");
         require(len == sv->len, "expected " + std::to_string(sv->len) + " amplitudes, got " + std::to_string(len));
+        sv->zero_pending = false;  // every amplitude is written below
         sv->bind();
         const uint64_t chunk = std::min<uint64_t>(len, uint64_t(1) << 24);
         DBuf stage(chunk * sizeof(double2));
@@ -1528,7 +1545,7 @@ int qvd_plan_ops(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t
 
 int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options *options, qvd_stats *stats) {
     return guard([&] {
-        check_handle(sv);
+        require(sv != nullptr, "null state handle");
         const double t0 = now();
         qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
         if (options) opt = *options;
@@ -1538,9 +1555,14 @@ int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_
         qvd_stats st{};
         st.gates = num_ops;
         st.plan_variant = variant;
+        // A pending |0...0> is made by the first stage when the plan starts
+        // with a register-tile stage, else written first.
+        const bool from_zero = sv->zero_pending && !steps.empty() && steps[0].is_stage && steps[0].stage.reg_tile;
+        if (!from_zero) materialize(sv);
+        sv->zero_pending = false;
         sv->bind();
         with_type(sv, [&](auto *s) {
-            run_plan(sv, s, steps, st, opt.flags);
+            run_plan(sv, s, steps, st, opt.flags, from_zero);
             return 0;
         });
         const double t2 = now();

@@ -905,7 +905,7 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
 
 }  // namespace
 
-template <typename T2, int WB, int RB, int FB>
+template <typename T2, int WB, int RB, int FB, bool MAY_ZERO>
 __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const VStageDesc *__restrict__ descs,
                                                      uint32_t ndesc, const uint32_t *__restrict__ block_start,
                                                      const VInstr *__restrict__ prog, const T2 *__restrict__ coefs,
@@ -927,7 +927,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage(T2 *__restrict__ s, const V
         nb = cbs[di + 1] - cbs[di];
     }
     const QVD_CONST VStageDesc &d = ((const QVD_CONST VStageDesc *)descs)[di];
-    stage_tiles<T2, WB, RB, FB, false>(s + d.state_off, d, bl, d.num_tiles, nb, prog, coefs, ldsops, ints);
+    stage_tiles<T2, WB, RB, FB, MAY_ZERO>(s + d.state_off, d, bl, d.num_tiles, nb, prog, coefs, ldsops, ints);
 }
 
 // Every stage of a batch of states in one launch. Items (runs of tiles of one
@@ -1026,13 +1026,20 @@ size_t vstage_lds_bytes() {
 template <typename T2>
 hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const uint32_t *block_start,
                          const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
-                         hipStream_t st) {
+                         hipStream_t st, bool from_zero) {
     const size_t lds = vstage_lds_bytes<T2>();
     using Kernel = void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *, const T2 *,
                             const StageOp *, const uint32_t *);
-    Kernel k;
-    if constexpr (sizeof(T2) == 8) k = k_vstage<T2, 4, 4, 4>;
-    else k = k_vstage<T2, 3, 4, 3>;
+    // A stage that may start from |0...0> has a kernel of its own, so the
+    // others keep the code without that branch.
+    Kernel k, kz;
+    if constexpr (sizeof(T2) == 8) {
+        k = k_vstage<T2, 4, 4, 4, false>;
+        kz = k_vstage<T2, 4, 4, 4, true>;
+    } else {
+        k = k_vstage<T2, 3, 4, 3, false>;
+        kz = k_vstage<T2, 3, 4, 3, true>;
+    }
     // The LDS size attribute is set once per device, before any launch a
     // caller may capture into a graph.
     static std::atomic<uint64_t> attr_set{0};
@@ -1043,8 +1050,11 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
     if (!(attr_set.load() & bit)) {
         e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)kz, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
         attr_set.fetch_or(bit);
     }
+    if (from_zero) k = kz;
     k<<<grid, 64 << vstage_geom<T2>().w_bits, lds, st>>>(s, descs, ndesc, block_start, prog, coefs, ldsops, ints);
     return hipGetLastError();
 }
@@ -1107,9 +1117,11 @@ template VGeom vstage_geom<double2>();
 template size_t vstage_lds_bytes<float2>();
 template size_t vstage_lds_bytes<double2>();
 template hipError_t launch_vstage<float2>(float2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
-                                          const float2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t);
+                                          const float2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t,
+                                          bool);
 template hipError_t launch_vstage<double2>(double2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
-                                           const double2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t);
+                                           const double2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t,
+                                           bool);
 template hipError_t launch_vstage_queue<float2>(float2 *, const VStageDesc *, const VQueueItem *, uint32_t, uint32_t *,
                                                 const VInstr *, const float2 *, const StageOp *, const uint32_t *,
                                                 unsigned, hipStream_t);
