@@ -112,6 +112,27 @@ std::vector<size_t> extract(const std::vector<Op> &ops, const std::vector<char> 
     return taken;
 }
 
+// extract over qubit masks (bit q = qubit q): the same ops in the same
+// order, without building each op's qubit list on every pass. can_run takes
+// the op's index.
+template <typename F>
+std::vector<size_t> extract_masked(const std::vector<uint64_t> &qmask, const std::vector<char> &done, F &&can_run) {
+    std::vector<size_t> taken;
+    uint64_t blocked = 0;
+    for (size_t i = 0; i < qmask.size(); i++) {
+        if (done[i]) continue;
+        if (!(qmask[i] & blocked) && can_run(i)) taken.push_back(i);
+        else blocked |= qmask[i];
+    }
+    return taken;
+}
+
+uint64_t mask_of(const std::vector<uint32_t> &qs) {
+    uint64_t m = 0;
+    for (uint32_t q : qs) m |= uint64_t(1) << q;
+    return m;
+}
+
 }  // namespace
 
 std::vector<Step> plan_unblocked(const std::vector<Op> &ops, uint32_t max_fused) {
@@ -132,21 +153,29 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
     if (ops.empty()) return steps;
     const uint32_t L = std::min(p.tile_bits, n);
     const uint32_t contig = std::min(p.contig, L);
+    require(n <= 64, "internal: the planner's qubit masks hold 64 qubits");
     std::vector<char> is_local(n, 0);
     std::vector<uint32_t> local;
     for (uint32_t q = 0; q < L; q++) {
         local.push_back(q);
         is_local[q] = 1;
     }
+    // Every op's qubits and targets as masks, and whether it may run in a
+    // tile at all (diagonal, or dense and narrow enough for the in-tile
+    // kernel).
+    std::vector<uint64_t> qmask(ops.size()), tmask(ops.size());
+    std::vector<char> stageable(ops.size());
+    for (size_t i = 0; i < ops.size(); i++) {
+        qmask[i] = mask_of(ops[i].qubits());
+        tmask[i] = mask_of(ops[i].targets);
+        stageable[i] = ops[i].diagonal || ops[i].targets.size() <= p.max_stage_op;
+    }
+    const uint64_t low = contig >= 64 ? ~uint64_t(0) : (uint64_t(1) << contig) - 1;
     // A dense op runs inside a tile when its targets are tile qubits and it
     // is narrow enough for the in-tile kernel; a diagonal op always runs
     // (its non-tile targets and every non-tile control are fixed per tile).
-    auto runs_locally = [&](const Op &op) {
-        if (op.diagonal) return true;
-        if (op.targets.size() > p.max_stage_op) return false;
-        for (uint32_t q : op.targets)
-            if (!is_local[q]) return false;
-        return true;
+    auto runs_under = [&](size_t i, uint64_t tile) {
+        return ops[i].diagonal || (stageable[i] && (tmask[i] & ~tile) == 0);
     };
     // Coupling graph of the circuit (qubits that share an op), for the
     // region candidates (variant bit 4 clear).
@@ -165,7 +194,8 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
     std::vector<char> done(ops.size(), 0);
     size_t remaining = ops.size();
     while (remaining > 0) {
-        std::vector<size_t> taken = extract(ops, done, runs_locally);
+        const uint64_t local_mask = mask_of(local);
+        std::vector<size_t> taken = extract_masked(qmask, done, [&](size_t i) { return runs_under(i, local_mask); });
         if (!taken.empty()) {
             std::vector<Op> stage_ops;
             for (size_t i : taken) {
@@ -198,18 +228,19 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
         // need, within the tile's free capacity.
         const uint32_t full_cap = L - contig;
         auto wanted_set = [&](uint32_t capacity) {
-            std::vector<uint32_t> wanted;
-            extract(ops, done, [&](const Op &op) {
-                if (op.diagonal) return true;
-                if (op.targets.size() > p.max_stage_op) return false;
-                std::vector<uint32_t> fresh;
-                for (uint32_t q : op.targets)
-                    if (q >= contig && !contains(wanted, q)) fresh.push_back(q);
-                if (wanted.size() + fresh.size() > capacity) return false;
-                wanted.insert(wanted.end(), fresh.begin(), fresh.end());
+            uint64_t wanted = 0;
+            extract_masked(qmask, done, [&](size_t i) {
+                if (ops[i].diagonal) return true;
+                if (!stageable[i]) return false;
+                const uint64_t fresh = tmask[i] & ~low & ~wanted;
+                if ((uint32_t)(__builtin_popcountll(wanted) + __builtin_popcountll(fresh)) > capacity) return false;
+                wanted |= fresh;
                 return true;
             });
-            return wanted;
+            std::vector<uint32_t> qs;
+            for (uint32_t q = 0; q < n; q++)
+                if ((wanted >> q) & 1) qs.push_back(q);
+            return qs;
         };
         auto tile_of = [&](const std::vector<uint32_t> &wanted) {
             std::vector<uint32_t> next;
@@ -225,15 +256,8 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
         if (p.variant & 4) {
             // The candidate tile that runs the most pending ops next.
             auto runnable_count = [&](const std::vector<uint32_t> &tile) {
-                std::vector<char> loc(n, 0);
-                for (uint32_t q : tile) loc[q] = 1;
-                return extract(ops, done, [&](const Op &op) {
-                           if (op.diagonal) return true;
-                           if (op.targets.size() > p.max_stage_op) return false;
-                           for (uint32_t q : op.targets)
-                               if (!loc[q]) return false;
-                           return true;
-                       }).size();
+                const uint64_t tm = mask_of(tile);
+                return extract_masked(qmask, done, [&](size_t i) { return runs_under(i, tm); }).size();
             };
             std::vector<std::vector<uint32_t>> cands;
             for (uint32_t c = full_cap; c >= 1 && c + 4 > full_cap; c--) cands.push_back(tile_of(wanted_set(c)));
@@ -275,15 +299,8 @@ std::vector<Step> plan_heuristic(const std::vector<Op> &ops, const PlanParams &p
             // ops run now, so the tiles sweep the circuit's light cones
             // instead of chasing its earliest pending op.
             auto runnable = [&](const std::vector<uint32_t> &tile) {
-                std::vector<char> loc(n, 0);
-                for (uint32_t q : tile) loc[q] = 1;
-                return extract(ops, done, [&](const Op &op) {
-                           if (op.diagonal) return true;
-                           if (op.targets.size() > p.max_stage_op) return false;
-                           for (uint32_t q : op.targets)
-                               if (!loc[q]) return false;
-                           return true;
-                       }).size();
+                const uint64_t tm = mask_of(tile);
+                return extract_masked(qmask, done, [&](size_t i) { return runs_under(i, tm); }).size();
             };
             std::vector<std::vector<uint32_t>> cands{next};
             for (uint32_t seed = contig; seed < n; seed++) {
