@@ -424,8 +424,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_compare(const TA *__restrict__ 
 // per-bit sums Q[q] = sum p [bit q] (q < B) and the per-pair sums
 // R[q] = sum p [bit q xor bit q+1] (q < B-1). Bits above B are constant in a
 // segment and are resolved by k_expect_combine.
-constexpr int EZ_TID_BITS = 8;   // bits from the thread index
-constexpr int EZ_MAX_M_BITS = 8; // bits from the per-thread loop
+constexpr int EZ_TID_BITS = 10;  // bits from the thread index (1024 threads)
+constexpr int EZ_MAX_M_BITS = 6; // bits from the per-thread loop
 
 // The descriptor of block b: `one` when there is a single state, else the
 // entry i of `descs` with start[i] <= b < start[i + 1] (binary search).
@@ -441,11 +441,12 @@ __device__ __forceinline__ uint32_t find_range(const S *__restrict__ start, uint
 }
 
 template <typename T2>
-__global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ s_all, ExpDesc one,
+__global__ __launch_bounds__(1 << EZ_TID_BITS) void k_expect_segments(const T2 *__restrict__ s_all, ExpDesc one,
                                                          const ExpDesc *__restrict__ descs, uint32_t ndesc,
                                                          const uint64_t *__restrict__ seg_start,
                                                          double *__restrict__ out_all) {
-    __shared__ double scratch[4];
+    constexpr int MAXV = 2 * (EZ_TID_BITS + EZ_MAX_M_BITS);  // values per segment, at most
+    __shared__ double scratch[(1 << EZ_TID_BITS) / 64][MAXV];
     ExpDesc d = one;
     uint64_t seg = blockIdx.x;
     if (ndesc > 1) {
@@ -465,8 +466,8 @@ __global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ 
     for (int j = 0; j < EZ_MAX_M_BITS; j++) S[j] = P[j] = 0;
     if (active) {
         const uint64_t base = (seg << B) + tid;
-        for (uint32_t m = 0; m < (1u << mb); m++) {
-            const double p = prob(s[base + ((uint64_t)m << tb)]);
+        const uint32_t M = 1u << mb;
+        auto add = [&](uint32_t m, double p) {
             T += p;
 #pragma unroll
             for (int j = 0; j < EZ_MAX_M_BITS; j++) {
@@ -476,41 +477,62 @@ __global__ __launch_bounds__(256) void k_expect_segments(const T2 *__restrict__ 
                     if ((uint32_t)j + 1 < mb && (bj ^ ((m >> (j + 1)) & 1))) P[j] += p;
                 }
             }
+        };
+        // U loads in flight per thread (M is a power of two, so a multiple
+        // of U once it reaches U: the loads need no bounds test, which would
+        // put each in a branch of its own and wait for it), then the sums in
+        // the order of m.
+        constexpr uint32_t U = 8;
+        if (M >= U) {
+            for (uint32_t m0 = 0; m0 < M; m0 += U) {
+                T2 a[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) a[u] = s[base + ((uint64_t)(m0 + u) << tb)];
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) add(m0 + u, prob(a[u]));
+            }
+        } else {
+            for (uint32_t m = 0; m < M; m++) add(m, prob(s[base + ((uint64_t)m << tb)]));
         }
     }
-    double *o = out + seg * (2 * B);
-    // T
-    double r = block_sum(T, scratch);
-    if (tid == 0) o[0] = r;
-    // Q[q], q < tb: T times the thread's bit q; q >= tb: S[q - tb].
-    for (uint32_t q = 0; q < B; q++) {
-        double v;
-        if (q < tb) v = ((tid >> q) & 1) ? T : 0.0;
-        else {
-            v = 0;
+    // The segment's 2B values: T, Q[q] (q < B) and R[q] (q < B - 1), each
+    // summed over the wave, then over the waves in order by one thread per
+    // value after a single barrier.
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    auto value = [&](uint32_t k) -> double {
+        if (k == 0) return T;
+        if (k <= B) {  // Q[q]: T times the thread's bit q for q < tb, else S[q - tb]
+            const uint32_t q = k - 1;
+            if (q < tb) return ((tid >> q) & 1) ? T : 0.0;
+            double v = 0;
 #pragma unroll
             for (int j = 0; j < EZ_MAX_M_BITS; j++)
                 if ((uint32_t)j == q - tb) v = S[j];
+            return v;
         }
-        r = block_sum(v, scratch);
-        if (tid == 0) o[1 + q] = r;
-    }
-    // R[q], q < B-1.
-    for (uint32_t q = 0; q + 1 < B; q++) {
-        double v;
-        if (q + 1 < tb) v = (((tid >> q) ^ (tid >> (q + 1))) & 1) ? T : 0.0;
-        else if (q + 1 == tb) {
+        const uint32_t q = k - 1 - B;  // R[q]
+        if (q + 1 < tb) return (((tid >> q) ^ (tid >> (q + 1))) & 1) ? T : 0.0;
+        if (q + 1 == tb) {
             // bit q from the thread, bit q+1 = loop bit 0.
             const double s0 = mb > 0 ? S[0] : 0.0;
-            v = ((tid >> q) & 1) ? T - s0 : s0;
-        } else {
-            v = 0;
-#pragma unroll
-            for (int j = 0; j < EZ_MAX_M_BITS; j++)
-                if ((uint32_t)j == q - tb) v = P[j];
+            return ((tid >> q) & 1) ? T - s0 : s0;
         }
-        r = block_sum(v, scratch);
-        if (tid == 0) o[1 + B + q] = r;
+        double v = 0;
+#pragma unroll
+        for (int j = 0; j < EZ_MAX_M_BITS; j++)
+            if ((uint32_t)j == q - tb) v = P[j];
+        return v;
+    };
+    const uint32_t nv = 2 * B;  // T, B values Q, B - 1 values R
+    for (uint32_t k = 0; k < nv; k++) {
+        const double w = wave_sum(value(k));
+        if (lane == 0) scratch[wave][k] = w;
+    }
+    __syncthreads();
+    if (tid < nv) {
+        double r = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += scratch[w][tid];
+        out[seg * (2 * B) + tid] = r;
     }
 }
 
@@ -827,7 +849,7 @@ ExpDesc expect_desc(uint32_t state_qubits, uint32_t n) {
 template <typename T2>
 void launch_expect_z(const T2 *s, uint32_t n, double *seg_sums, double *z, double *zz, hipStream_t st) {
     const ExpDesc d = expect_desc(n, n);
-    k_expect_segments<T2><<<(unsigned)d.nsegs, 256, 0, st>>>(s, d, nullptr, 1, nullptr, seg_sums);
+    k_expect_segments<T2><<<(unsigned)d.nsegs, 1 << EZ_TID_BITS, 0, st>>>(s, d, nullptr, 1, nullptr, seg_sums);
     const uint32_t outs = 2 * n - 1;
     k_expect_combine<<<(outs + 63) / 64, 64, 0, st>>>(seg_sums, d, nullptr, 1, nullptr, z, zz);
 }
@@ -837,7 +859,7 @@ void launch_expect_batch(const T2 *s, const ExpDesc *descs, uint32_t ndesc, cons
                          uint64_t total_segs, const uint32_t *out_start, uint32_t total_outs, double *seg_sums,
                          double *z, double *zz, hipStream_t st) {
     const ExpDesc none{};
-    k_expect_segments<T2><<<(unsigned)total_segs, 256, 0, st>>>(s, none, descs, ndesc, seg_start, seg_sums);
+    k_expect_segments<T2><<<(unsigned)total_segs, 1 << EZ_TID_BITS, 0, st>>>(s, none, descs, ndesc, seg_start, seg_sums);
     k_expect_combine<<<(total_outs + 63) / 64, 64, 0, st>>>(seg_sums, none, descs, ndesc, out_start, z, zz);
 }
 
