@@ -21,11 +21,16 @@ Plans timed on the same circuit and state:
                 variant 24)
 
 Printed as one JSON line (stdout):
-  value          seconds per circuit of the learned plan (lower is better)
-  roofline       the tile-stage kernel of that plan: algorithmic bytes per
-                 launch (one read and one write of the 8 GiB state) over its
-                 average duration, timed with HIP events on the engine's
-                 stream in the timed steps, against the 8 TB/s HBM peak
+  value          seconds per circuit of the learned plan (lower is better);
+                 its stages skip the tiles known to be zero (qubits no
+                 non-diagonal gate has touched yet since |0...0>: the first
+                 stage computes one tile and the reset writes the rest);
+                 all_tiles_seconds is the same plan visiting every tile
+  roofline       the tile-stage kernel of that plan: its algorithmic bytes
+                 (each stage reads and writes the tiles it visits) over the
+                 stage kernels' summed duration, timed with HIP events on the
+                 engine's stream in the timed steps, against the 8 TB/s HBM
+                 peak
   error          complex64 result against the oracle's FP64 engine on the
                  host (and against the same circuit in complex128 on the GPU)
   cpu_baseline   the oracle's restatement of the reference CPU path (AVX2
@@ -110,7 +115,8 @@ def time_plan(sv, ops, steps: int, warmup: int, **kw) -> dict:
         launches += st["stages"]
     sec = (time.perf_counter() - t0) / steps
     return {"seconds": sec, "stage_kernel_seconds": stage_s / steps, "other_kernel_seconds": other_s / steps,
-            "plan_seconds": plan_s / steps, "stage_launch_seconds": stage_s / max(launches, 1), "stats": st}
+            "plan_seconds": plan_s / steps, "stage_launch_seconds": stage_s / max(launches, 1), "stats": st,
+            "stage_bytes": st["stage_bytes"]}
 
 
 def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
@@ -147,13 +153,14 @@ def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
     with Batch(circuits, "single") as d:
         default_s, _ = timed(d)
         default_stages = sum(d.stages())
+    with Batch(circuits, "single", all_tiles=True) as a:
+        all_tiles_s, _ = timed(a)
     geometry = {str(n): sorted({f for f, m in zip(lanes, sizes) if m == n}) for n in sorted(set(sizes))}
-    # Algorithmic HBM bytes of one replay: each circuit's state (widened to
-    # at least one 14-qubit tile) is written by its first stage (made from
-    # |0...0> in registers, not read), read and written by every later stage
-    # and read by the expectation values: 2 x stages passes of 8 bytes per
-    # amplitude.
-    passes_bytes = sum(2 * k * 8 * (1 << max(n, 14)) for n, k in zip(sizes, stage_counts))
+    # Algorithmic HBM bytes of one replay, as the engine counts them: each
+    # stage reads and writes the tiles it visits (a first stage from |0...0>
+    # only writes), the initialisation (when a first stage skips zero tiles)
+    # writes every state and the expectation values read every state.
+    passes_bytes = stats["stage_bytes"] + stats["other_bytes"]
     achieved = passes_bytes / batch_s / 1e9
     # the same circuits one by one on the GPU: plan, run and expectation values per state
     t = time.perf_counter()
@@ -193,14 +200,16 @@ def run_batch(count: int, cpu_sample: int, threads: int) -> dict:
                "sample": f"the first {k} of the {count} circuits one by one (oracle apply + <Z>, <ZZ> from the "
                          f"probabilities), scaled to {count}", "max_abs_diff_vs_batch": cworst}
     log(f"batch of {count} HEA circuits: {batch_s * 1e3:.2f} ms per run, one stage launch ({stats['stages']} "
-        f"stages, at most {stats['passes']} per circuit; {achieved:.0f} GB/s algorithmic; build and autotune "
-        f"{build_s:.2f} s; default geometry {default_s * 1e3:.2f} ms, {default_stages} stages); "
+        f"stages, at most {stats['passes']} per circuit, {stats['skipped_tiles']} zero tiles skipped; {achieved:.0f} "
+        f"GB/s algorithmic; build and autotune {build_s:.2f} s; default geometry {default_s * 1e3:.2f} ms, "
+        f"{default_stages} stages; every tile visited {all_tiles_s * 1e3:.2f} ms); "
         f"one by one on the GPU {single_s:.3f} s; max |diff| {worst:.2e}"
         + (f"; CPU one by one {cpu['value']:.2f} s (sampled)" if cpu else ""))
     return {"circuits": count, "qubits": "12..22", "seconds": batch_s, "one_by_one_seconds": single_s,
             "speedup": single_s / batch_s, "stage_launches_per_replay": 1, "stages": stats["stages"],
             "max_stages_per_circuit": stats["passes"], "build_and_autotune_seconds": build_s,
             "fixed_lanes_by_qubits": geometry, "default_geometry_seconds": default_s,
+            "skipped_tiles": stats["skipped_tiles"], "all_tiles_seconds": all_tiles_s,
             "default_geometry_stages": default_stages, "max_abs_diff_vs_one_by_one": worst,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_replay": passes_bytes},
@@ -350,6 +359,10 @@ def main() -> int:
     la = time_plan(sv, ops, args.steps, min(args.warmup, 1), variant=24, **kw)
     plans["lookahead"] = la
     log(f"lookahead (variant 24): {la['seconds']:.4f} s/circuit ({la['stats']['stages']} stages)")
+    at = time_plan(sv, ops, args.steps, min(args.warmup, 1), all_tiles=True, **kw)
+    plans["heuristic_all_tiles"] = at
+    log(f"heuristic, every tile visited: {at['seconds']:.4f} s/circuit (stage kernels {at['stage_kernel_seconds']:.4f} s; "
+        f"with zero tiles skipped {h['stats']['skipped_tiles']} tiles are not visited)")
 
     # The learned planner (a trained cost model in the tree): over the plans
     # that reproduce the heuristic's amplitudes (the headline) and over every
@@ -372,12 +385,13 @@ def main() -> int:
     head = plans["learned"] if have_model else h
     value = head["seconds"]
     st = head["stats"]
-    state_bytes = (1 << n) * 8
-    per_launch_bytes = 2 * state_bytes
-    achieved = per_launch_bytes / head["stage_launch_seconds"] / 1e9
+    achieved = head["stage_bytes"] / head["stage_kernel_seconds"] / 1e9
 
     result_extra = {"passes": st["passes"], "stages": st["stages"], "fused_ops": st["kernels"], "gates": st["gates"],
-                    "stage_launch_ms": head["stage_launch_seconds"] * 1e3, "plan_seconds": head["plan_seconds"]}
+                    "stage_launch_ms": head["stage_launch_seconds"] * 1e3, "plan_seconds": head["plan_seconds"],
+                    "stage_bytes_per_circuit": head["stage_bytes"], "skipped_tiles": st["skipped_tiles"],
+                    "all_tiles_seconds": at["seconds"],
+                    "all_tiles_roofline_achieved": at["stage_bytes"] / at["stage_kernel_seconds"] / 1e9}
 
     for _ in range(args.calibration_passes):
         sv.scale(1.0)
@@ -489,7 +503,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "error": error,
         "plans": {k: {"seconds": v["seconds"], "variant": v["stats"]["plan_variant"], "stages": v["stats"]["stages"],
-                      "stage_launch_ms": v["stage_launch_seconds"] * 1e3,
+                      "stage_launch_ms": v["stage_launch_seconds"] * 1e3, "skipped_tiles": v["stats"]["skipped_tiles"],
                       **({"max_abs_diff_vs_heuristic": diffs[k]["max_abs"]} if k in diffs else {})}
                   for k, v in plans.items()},
     }

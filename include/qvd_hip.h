@@ -98,6 +98,15 @@ typedef struct qvd_options {
  * geometry, and the fastest is kept, for this batch and for later batches of
  * the process with that class. New in this engine. */
 #define QVD_FLAG_AUTOTUNE 4u
+/* qvd_options.flags: visit every register tile. By default a stage skips the
+ * tiles known to be zero: a qubit that no non-diagonal op has targeted since
+ * the state was |0...0> (qvd_sv_create, qvd_sv_set_zero, a batch's start)
+ * is still |0>, so every amplitude with its bit set is zero, and a stage
+ * whose tile leaves that qubit out neither reads nor writes those tiles (its
+ * ops are diagonal or controlled on that qubit, and keep them zero). The
+ * amplitudes are the same either way, except that a zero may come out as
+ * -0.0 when computed. */
+#define QVD_FLAG_ALL_TILES 8u
 
 /* What a run did: simulator::Stats (src/simulator.rs:70-93). */
 typedef struct qvd_stats {
@@ -111,6 +120,11 @@ typedef struct qvd_stats {
     double stage_kernel_seconds;  /* sum of tile-stage kernel durations (QVD_FLAG_TIMING) */
     double other_kernel_seconds;  /* sum of full-state op kernel durations (QVD_FLAG_TIMING) */
     uint64_t plan_variant;        /* the plan variant that ran */
+    uint64_t stage_bytes;         /* algorithmic HBM bytes of the register-tile stages: each reads and writes
+                                     the tiles it visits (a stage from |0...0> only writes) */
+    uint64_t other_bytes;         /* algorithmic HBM bytes of the other passes (full-state ops, writing a
+                                     pending |0...0>, a batch's initialisation and expectation values) */
+    uint64_t skipped_tiles;       /* register tiles not visited because they are known to be zero */
 } qvd_stats;
 
 const char *qvd_last_error(void);

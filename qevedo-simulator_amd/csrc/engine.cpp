@@ -95,6 +95,7 @@ struct qvd_sv {
     double last_sample_seconds = 0;
     std::vector<double> stage_seconds;  // register-tile stage kernels of the last timed apply_ops
     bool zero_pending = false;          // the state is |0...0> but the amplitudes are not written yet
+    uint64_t zero_qubits = 0;           // qubits known to be |0> (every amplitude with the bit set is zero)
 
     size_t elem() const { return precision == QVD_C64 ? sizeof(float2) : sizeof(double2); }
     void bind() const { QVD_HIP(hipSetDevice(device)); }
@@ -124,6 +125,8 @@ struct qvd_batch {
     std::vector<uint32_t> qubits;     // each circuit's qubit count
     std::vector<uint32_t> fixed;      // each circuit's register-tile geometry (fixed lane bits)
     std::vector<uint32_t> circuit_stages;  // each circuit's number of stages
+    bool init_states = false;         // a first stage skips zero tiles: the replay writes every state first
+    uint64_t stage_bytes = 0, other_bytes = 0, skipped_tiles = 0;  // per replay
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~qvd_batch() {
@@ -163,10 +166,37 @@ void materialize(qvd_sv *sv) {
     sv->zero_pending = false;
 }
 
-// Every entry point that uses a state's amplitudes starts here.
+// Every entry point that uses a state's amplitudes starts here. It may change
+// them, so the qubits known to be |0> are forgotten (qvd_sv_apply_ops keeps
+// track of them itself).
 void check_handle(const qvd_sv *sv) {
     require(sv != nullptr, "null state handle");
     materialize(const_cast<qvd_sv *>(sv));
+    const_cast<qvd_sv *>(sv)->zero_qubits = 0;
+}
+
+uint64_t qubit_mask(uint32_t n) { return n >= 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1; }
+
+// The qubits still known to be |0> after an op: a non-diagonal op may set
+// its targets; a diagonal op, and a control, leave every bit as it is.
+uint64_t after_op(uint64_t zero, const Op &op) {
+    if (!op.diagonal)
+        for (uint32_t q : op.targets) zero &= ~(uint64_t(1) << q);
+    return zero;
+}
+
+// Drops from a stage's tile index the non-tile qubits known to be |0>: a tile
+// with such a bit set holds only zeros, and the stage (dense ops on tile
+// qubits only; diagonals and controls on the others) keeps them zero, so the
+// stage neither reads nor writes it. Returns the number of tiles dropped.
+uint64_t drop_zero_tiles(VStageDesc &d, uint64_t zero) {
+    uint32_t k = 0;
+    for (uint32_t j = 0; j < d.nglob; j++)
+        if (!((zero >> d.glob_pos[j]) & 1)) d.glob_pos[k++] = d.glob_pos[j];
+    const uint64_t before = d.num_tiles;
+    d.nglob = k;
+    d.num_tiles = uint64_t(1) << k;
+    return before - d.num_tiles;
 }
 
 // Validates qubits of an op the way the reference asserts them
@@ -442,9 +472,10 @@ struct LaunchTimer {
     }
 };
 
+// Runs a plan on the state. zero: the qubits known to be |0> before it (all
+// of them when the state is a pending |0...0>); returns those after it.
 template <typename T2>
-void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags,
-              bool from_zero = false) {
+uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags, uint64_t zero) {
     const bool timing = (flags & QVD_FLAG_TIMING) != 0;
     LaunchTimer timer;
     timer.on = timing;
@@ -462,21 +493,41 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
     std::vector<VStageDesc> vdescs;
     PendingDiag pend(sv->n);
     const VGeom geom = vstage_geom<T2>();
+    const bool skip_zero = (flags & QVD_FLAG_ALL_TILES) == 0;
+    const uint64_t state_bytes = sv->len * sizeof(T2);
+    // A pending |0...0> is made in registers by the first step when that is a
+    // register-tile stage visiting every tile; otherwise it is written first.
+    bool write_zeros = sv->zero_pending;
     for (const Step &step : steps) {
-        if (!step.is_stage) continue;
+        if (!step.is_stage) {
+            zero = after_op(zero, step.op);
+            st.other_bytes += 2 * state_bytes;
+            continue;
+        }
         if (step.stage.reg_tile) {
             VProgram p = build_vprogram(step.stage, sv->n, geom, pend, vco, vlds, vints, true);
             require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
             p.desc.instr_off = (uint32_t)vins.size();
-            // The caller passes from_zero only when this is the plan's first step.
-            p.desc.from_zero = from_zero && vdescs.empty() && &step == &steps[0];
+            const uint64_t dropped = skip_zero ? drop_zero_tiles(p.desc, zero) : 0;
+            st.skipped_tiles += dropped;
+            if (sv->zero_pending && &step == &steps[0] && dropped == 0) {
+                p.desc.from_zero = 1;
+                write_zeros = false;
+            }
+            st.stage_bytes += (p.desc.from_zero ? 1 : 2) * (sizeof(T2) << p.desc.L) * p.desc.num_tiles;
             vins.insert(vins.end(), p.instr.begin(), p.instr.end());
             vdescs.push_back(p.desc);
-            continue;
+        } else {
+            StageDesc d;
+            lower_stage<T2>(step.stage, sv->n, sops, coefs, d);
+            descs.push_back(d);
+            st.stage_bytes += 2 * state_bytes;
         }
-        StageDesc d;
-        lower_stage<T2>(step.stage, sv->n, sops, coefs, d);
-        descs.push_back(d);
+        for (const Op &op : step.stage.ops) zero = after_op(zero, op);
+    }
+    if (write_zeros) {
+        materialize(sv);
+        st.other_bytes += state_bytes;
     }
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
@@ -491,6 +542,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
     for (const Step &step : steps)
         if (!step.is_stage) o_full.push_back(up.put(full_coefficients<T2>(step.op)));
     char *base = up.to_device(sv);
+    sv->zero_pending = false;  // written above, or made by the first stage below
     const StageOp *dops = reinterpret_cast<const StageOp *>(base + o_ops);
     const T2 *dcoef = reinterpret_cast<const T2 *>(base + o_coef);
     const VInstr *dvins = reinterpret_cast<const VInstr *>(base + o_vins);
@@ -551,6 +603,7 @@ void run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, 
             sv->stage_seconds.push_back(ms * 1e-3);
         }
     }
+    return zero;
 }
 
 uint64_t state_cap_bytes() {
@@ -1017,6 +1070,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
         std::vector<Step> steps = make_plan(npad, b->precision, ci.ops, ci.num_ops, opt, nullptr, gc.f_bits);
         PendingDiag pend(npad);
         state_off[c] = off;
+        // Every circuit starts from |0...0>: its stages skip the tiles known
+        // to be zero (a circuit whose first stage does must have its state
+        // written with zeros first).
+        uint64_t zero = (opt.flags & QVD_FLAG_ALL_TILES) ? 0 : qubit_mask(npad);
         for (const Step &st : steps) {
             if (!st.is_stage || !st.stage.reg_tile)
                 fail(QVD_EUNSUPPORTED, "circuit " + std::to_string(c) +
@@ -1025,6 +1082,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
             require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
             p.desc.instr_off = (uint32_t)vins.size();
             p.desc.state_off = off;
+            const uint64_t dropped = drop_zero_tiles(p.desc, zero);
+            b->skipped_tiles += dropped;
+            if (stage_descs[c].empty() && dropped) b->init_states = true;
+            for (const Op &op : st.stage.ops) zero = after_op(zero, op);
             vins.insert(vins.end(), p.instr.begin(), p.instr.end());
             stage_descs[c].push_back(p.desc);
             b->kernels += st.stage.ops.size();
@@ -1076,7 +1137,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
             if (r >= stage_descs[c].size()) continue;
             any = true;
             VStageDesc d = stage_descs[c][r];
-            d.from_zero = r == 0;
+            // The first stage makes its tiles from |0...0> unless the states
+            // are written first (a first stage that skips zero tiles).
+            d.from_zero = r == 0 && !b->init_states;
+            b->stage_bytes += (d.from_zero ? 1 : 2) * (sizeof(T2) << d.L) * d.num_tiles;
             const uint32_t di = (uint32_t)all_descs.size();
             all_descs.push_back(d);
             // Large items while much work remains (each item costs a fetch, a
@@ -1096,6 +1160,9 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
         b->levels++;
     }
     require(items.size() < (uint64_t(1) << 32), "batch too large for one stage queue");
+    // Initialisation (when it runs) writes every state; the expectation
+    // values read every state.
+    b->other_bytes = ((any_empty || b->init_states) ? 2 : 1) * off * sizeof(T2);
     b->nitems = (uint32_t)items.size();
     b->grid = (unsigned)std::min<uint64_t>(resident, std::max<uint64_t>(items.size(), 1));
     // Upload.
@@ -1129,9 +1196,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     // capture), then the capture of the same sequence.
     T2 *s = b->amps.as<T2>();
     auto enqueue = [&]() {
-        // Every circuit's first stage starts from |0...0> without loading; a
-        // circuit with no stage gets its state from the init.
-        if (any_empty) launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
+        // Every circuit's first stage starts from |0...0> without loading,
+        // unless some first stage skips zero tiles or a circuit has no stage:
+        // then every state is written first.
+        if (any_empty || b->init_states) launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
         launch_zero_words(b->sync.as<uint32_t>(), VQUEUE_COUNTERS + count, b->stream);
         QVD_HIP(hipGetLastError());
         if (b->nitems)
@@ -1409,6 +1477,7 @@ int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out) {
         QVD_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
         sv->amps.alloc(bytes);
         sv->zero_pending = true;
+        sv->zero_qubits = qubit_mask(num_qubits);
         *out = sv.release();
     });
 }
@@ -1443,6 +1512,7 @@ int qvd_sv_set_zero(qvd_sv *sv) {
     return guard([&] {
         require(sv != nullptr, "null state handle");
         sv->zero_pending = true;
+        sv->zero_qubits = qubit_mask(sv->n);
     });
 }
 
@@ -1454,6 +1524,7 @@ int qvd_sv_copy(qvd_sv *dst, const qvd_sv *src) {
         if (dst == src) return;
         // dst is overwritten whole: a pending |0...0> in src is copied as pending.
         dst->zero_pending = src->zero_pending;
+        dst->zero_qubits = src->zero_qubits;
         if (src->zero_pending) return;
         src->sync();
         dst->bind();
@@ -1469,6 +1540,7 @@ int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len) {
         require(amplitudes != nullptr, "null amplitudes");
         require(len == sv->len, "expected " + std::to_string(sv->len) + " amplitudes, got " + std::to_string(len));
         sv->zero_pending = false;  // every amplitude is written below
+        sv->zero_qubits = 0;
         sv->bind();
         const uint64_t chunk = std::min<uint64_t>(len, uint64_t(1) << 24);
         DBuf stage(chunk * sizeof(double2));
@@ -1555,16 +1627,13 @@ int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_
         qvd_stats st{};
         st.gates = num_ops;
         st.plan_variant = variant;
-        // A pending |0...0> is made by the first stage when the plan starts
-        // with a register-tile stage, else written first.
-        const bool from_zero = sv->zero_pending && !steps.empty() && steps[0].is_stage && steps[0].stage.reg_tile;
-        if (!from_zero) materialize(sv);
-        sv->zero_pending = false;
+        // An empty plan leaves a pending |0...0> pending.
         sv->bind();
-        with_type(sv, [&](auto *s) {
-            run_plan(sv, s, steps, st, opt.flags, from_zero);
-            return 0;
-        });
+        if (!steps.empty()) {
+            const uint64_t zero = sv->zero_pending ? qubit_mask(sv->n) : sv->zero_qubits;
+            sv->zero_qubits = 0;  // until the plan has run
+            sv->zero_qubits = with_type(sv, [&](auto *s) { return run_plan(sv, s, steps, st, opt.flags, zero); });
+        }
         const double t2 = now();
         st.plan_seconds = t1 - t0;
         st.gate_seconds = t2 - t0;
@@ -1947,6 +2016,9 @@ int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats) {
             st.passes = batch->levels;
             st.stages = batch->stages;
             st.gate_seconds = t1 - t0;
+            st.stage_bytes = batch->stage_bytes;
+            st.other_bytes = batch->other_bytes;
+            st.skipped_tiles = batch->skipped_tiles;
             *stats = st;
         }
     });

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("QVD_LIB", os.path.join(_PKG_ROOT, "lib", "libqvd_hip.
 QVD_OK, QVD_EINVAL, QVD_ENOMEM, QVD_EHIP, QVD_EUNSUPPORTED = 0, -1, -2, -3, -4
 C64, C128 = 0, 1
 PLANNER_HEURISTIC, PLANNER_LEARNED, PLANNER_LEARNED_ANY = 0, 1, 2
-FLAG_TIMING, FLAG_DENSE_VALU, FLAG_AUTOTUNE = 1, 2, 4
+FLAG_TIMING, FLAG_DENSE_VALU, FLAG_AUTOTUNE, FLAG_ALL_TILES = 1, 2, 4, 8
 
 # Every symbol include/qvd_hip.h declares.
 EXPORTS = (
@@ -82,6 +82,9 @@ class Stats(C.Structure):
         ("stage_kernel_seconds", C.c_double),
         ("other_kernel_seconds", C.c_double),
         ("plan_variant", C.c_uint64),
+        ("stage_bytes", C.c_uint64),
+        ("other_bytes", C.c_uint64),
+        ("skipped_tiles", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -355,16 +358,19 @@ class StateVector:
         _check(lib().qvd_sv_apply(self.h, _p(m.view(np.float64), C.c_double), _p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size))
 
     def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC,
-                  timing: bool = False, variant: int = 0, dense_valu: bool = False) -> dict:
+                  timing: bool = False, variant: int = 0, dense_valu: bool = False, all_tiles: bool = False) -> dict:
         """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default.
 
         planner PLANNER_LEARNED scores, with the installed cost model, the
         plan variants that run the heuristic's arithmetic byte for byte
         (bit-identical amplitudes); PLANNER_LEARNED_ANY scores every variant
         (other op orders round differently); both are the heuristic when no
-        model is installed. variant picks a heuristic variant directly."""
+        model is installed. variant picks a heuristic variant directly.
+        all_tiles: visit the register tiles known to be zero too
+        (QVD_FLAG_ALL_TILES)."""
         arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-        flags = (FLAG_TIMING if timing else 0) | (FLAG_DENSE_VALU if dense_valu else 0)
+        flags = ((FLAG_TIMING if timing else 0) | (FLAG_DENSE_VALU if dense_valu else 0)
+                 | (FLAG_ALL_TILES if all_tiles else 0))
         opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, flags, variant)
         st = Stats()
         _check(lib().qvd_sv_apply_ops(self.h, arr.arr, arr.n, C.byref(opt), C.byref(st)))
@@ -461,14 +467,16 @@ class Batch:
     neighbouring qubits).
     """
 
-    def __init__(self, circuits, precision: str = "single", max_fused_qubits: int = 4, autotune: bool = False):
+    def __init__(self, circuits, precision: str = "single", max_fused_qubits: int = 4, autotune: bool = False,
+                 all_tiles: bool = False):
         self.h = None
         self.sizes = [int(c[0]) for c in circuits]
         self.arrays = [c[1] if isinstance(c[1], OpArray) else OpArray(c[1]) for c in circuits]
         cs = (CircuitDesc * len(self.sizes))()
         for i, (c, arr) in enumerate(zip(circuits, self.arrays)):
             cs[i] = CircuitDesc(int(c[0]), arr.arr, arr.n, int(c[2]) if len(c) > 2 else 0)
-        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, FLAG_AUTOTUNE if autotune else 0, 0)
+        opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC,
+                      (FLAG_AUTOTUNE if autotune else 0) | (FLAG_ALL_TILES if all_tiles else 0), 0)
         h = C.c_void_p()
         _check(lib().qvd_batch_create(cs, len(self.sizes), {"single": C64, "double": C128}[precision], C.byref(opt),
                                       C.byref(h)))

@@ -71,7 +71,7 @@ def collect(circuits, variants=None, repeats: int = 2, log=print) -> tuple:
             states = {n: StateVector(n, "single")}
         sv = states[n]
         sv.set_zero()
-        sv.apply_ops(ops)
+        sv.apply_ops(ops, all_tiles=True)
         seen = set()
         for v in variants:
             f = plan_features(n, ops, "single", v)
@@ -81,8 +81,10 @@ def collect(circuits, variants=None, repeats: int = 2, log=print) -> tuple:
             seen.add(key)
             best = None
             for _ in range(repeats):
+                # every tile visited: the model prices whole stages, also a
+                # first stage that starts from |0...0>
                 sv.set_zero()
-                sv.apply_ops(ops, timing=True, variant=v)
+                sv.apply_ops(ops, timing=True, variant=v, all_tiles=True)
                 t = sv.stage_seconds()
                 if len(t) != len(f):
                     raise RuntimeError(f"{name} variant {v}: {len(t)} timed stages, {len(f)} feature rows")
