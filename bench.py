@@ -59,9 +59,10 @@ def log(msg: str) -> None:
 
 
 def measured_traffic(n: int):
-    """HBM bytes per stage launch of this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy pass
-    of known size), or None when no such measurement exists for this size."""
+    """HBM bytes per stage launch of this workload (the plan's measured
+    stage traffic over its launches), from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy pass of known
+    size), or None when no such measurement exists for this size."""
     path = os.path.join(ROOT, "profiles", "k_vstage_random30_c64_traffic.json")
     if n != 30 or not os.path.exists(path):
         return None
@@ -499,7 +500,8 @@ def main() -> int:
                    "tile_bits": args.tile_bits or "register tile, 2^14 amplitudes",
                    "planner": "learned (bit-identical to the heuristic)" if have_model else "heuristic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
+                     "algorithmic_bytes_per_launch": head["stage_bytes"] / max(st["stages"], 1)},
         "cpu_baseline": cpu,
         "error": error,
         "plans": {k: {"seconds": v["seconds"], "variant": v["stats"]["plan_variant"], "stages": v["stats"]["stages"],
