@@ -1362,14 +1362,24 @@ std::vector<uint32_t> batch_geometry(const qvd_circuit *circuits, uint32_t count
     return fixed;
 }
 
-// The sampler's buffers carved from one workspace (256-byte aligned parts);
-// returns the bytes needed, and fills `w`, the uniforms' and the outputs'
-// device addresses when `base` is not null.
-size_t carve_sample(const qvd_sv *sv, uint64_t shots, char *base, SampleScratch &w, double **du, uint64_t **dout) {
-    // Chunks of the reference's CHUNK_BLOCKS blocks (src/measure.rs:12-13):
-    // 4096 x 8 amplitudes in complex64, 4096 x 4 in complex128.
-    const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
+// The walk's marks: the running sum every 2^6 amplitudes (1/64 of the
+// amplitude count, in doubles), so a shot's resumed sum is at most 64 long.
+constexpr uint32_t SAMPLE_MARK_BITS = 6;
+
+// Chunks of the reference's CHUNK_BLOCKS blocks (src/measure.rs:12-13):
+// 4096 x 8 amplitudes in complex64, 4096 x 4 in complex128.
+uint64_t sample_chunk(const qvd_sv *sv) { return sv->precision == QVD_C64 ? 32768 : 16384; }
+
+// The sampler's buffers carved from one workspace (256-byte aligned parts),
+// with the walk's marks every 2^mark_bits amplitudes; returns the bytes
+// needed, and fills `w`, the uniforms' and the outputs' device addresses when
+// `base` is not null.
+size_t carve_sample(const qvd_sv *sv, uint64_t shots, uint32_t mark_bits, char *base, SampleScratch &w, double **du,
+                    uint64_t **dout) {
+    const uint64_t chunk = sample_chunk(sv);
     const uint64_t nc = (sv->len + chunk - 1) / chunk;
+    const uint64_t nmarks = (sv->len + (uint64_t(1) << mark_bits) - 1) >> mark_bits;
+    w.mark_bits = mark_bits;
     w.sort_temp_bytes = sort_temp_bytes(shots);
     require(w.sort_temp_bytes > 0, "radix sort temporary size query failed");
     size_t off = 0;
@@ -1382,7 +1392,12 @@ size_t carve_sample(const qvd_sv *sv, uint64_t shots, char *base, SampleScratch 
     const size_t o_bounds = carve((nc + 1) * sizeof(uint64_t)), o_tu = carve(shots * sizeof(double));
     const size_t o_ts = carve(shots * sizeof(double)), o_du = carve(shots * sizeof(double));
     const size_t o_out = carve(shots * sizeof(uint64_t)), o_temp = carve(w.sort_temp_bytes);
+    const size_t o_marks = carve(nmarks * sizeof(double)), o_walked = carve(nc * sizeof(uint32_t));
+    const size_t o_last = carve(nc * sizeof(uint64_t));
     if (base) {
+        w.marks = reinterpret_cast<double *>(base + o_marks);
+        w.walked = reinterpret_cast<uint32_t *>(base + o_walked);
+        w.last_nonzero = reinterpret_cast<uint64_t *>(base + o_last);
         w.sums = reinterpret_cast<double *>(base + o_sums);
         w.prefix = reinterpret_cast<double *>(base + o_prefix);
         w.bounds = reinterpret_cast<uint64_t *>(base + o_bounds);
@@ -1398,12 +1413,12 @@ size_t carve_sample(const qvd_sv *sv, uint64_t shots, char *base, SampleScratch 
 // Two-pass sampling with the buffers carved from `base` (at least
 // carve_sample bytes of device memory): copies the uniforms in, runs the
 // sampler on the handle's stream, copies the outcomes out and waits.
-void sample_with(qvd_sv *sv, const double *uniforms, uint64_t shots, uint64_t *out, char *base) {
-    const uint64_t chunk = sv->precision == QVD_C64 ? 32768 : 16384;
+void sample_with(qvd_sv *sv, const double *uniforms, uint64_t shots, uint32_t mark_bits, uint64_t *out, char *base) {
+    const uint64_t chunk = sample_chunk(sv);
     SampleScratch w;
     double *du = nullptr;
     uint64_t *dout = nullptr;
-    carve_sample(sv, shots, base, w, &du, &dout);
+    carve_sample(sv, shots, mark_bits, base, w, &du, &dout);
     QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
     with_type(sv, [&](auto *s) {
         QVD_HIP(launch_sample(s, sv->len, chunk, du, shots, w, dout, sv->stream));
@@ -1857,9 +1872,9 @@ int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, ui
         // One workspace per handle, grown to the largest call and kept: no
         // allocation on repeated calls.
         SampleScratch w;
-        const size_t need = carve_sample(sv, shots, nullptr, w, nullptr, nullptr);
+        const size_t need = carve_sample(sv, shots, SAMPLE_MARK_BITS, nullptr, w, nullptr, nullptr);
         if (sv->sample_ws.bytes < need) sv->sample_ws.alloc(need);
-        sample_with(sv, uniforms, shots, out, sv->sample_ws.as<char>());
+        sample_with(sv, uniforms, shots, SAMPLE_MARK_BITS, out, sv->sample_ws.as<char>());
         sv->last_sample_seconds = now() - t0;
     });
 }
@@ -1873,8 +1888,13 @@ int qvd_sv_sample_noalloc(const qvd_sv *sv_c, const double *uniforms, uint64_t s
         require(shots < (uint64_t(1) << 31), "at most 2^31 - 1 shots per call");
         sv->bind();
         const double t0 = now();
+        // The finest marks that fit the fixed workspace: coarser marks only
+        // lengthen each shot's resumed sum (at most one chunk).
         SampleScratch w;
-        const size_t need = carve_sample(sv, shots, nullptr, w, nullptr, nullptr);
+        uint32_t mark_bits = SAMPLE_MARK_BITS;
+        size_t need = carve_sample(sv, shots, mark_bits, nullptr, w, nullptr, nullptr);
+        while (need > NOALLOC_SCRATCH_BYTES && (uint64_t(2) << mark_bits) <= sample_chunk(sv))
+            need = carve_sample(sv, shots, ++mark_bits, nullptr, w, nullptr, nullptr);
         if (need > NOALLOC_SCRATCH_BYTES)
             fail(QVD_EINVAL, "qvd_sv_sample_noalloc: " + std::to_string(shots) + " shots on " + std::to_string(sv->n) +
                                  " qubits need " + std::to_string(need) + " bytes of workspace, more than its fixed " +
@@ -1885,7 +1905,7 @@ int qvd_sv_sample_noalloc(const qvd_sv *sv_c, const double *uniforms, uint64_t s
         // process are serialised here (the call returns after its GPU work).
         static std::mutex mu;
         std::lock_guard<std::mutex> lock(mu);
-        sample_with(sv, uniforms, shots, out, static_cast<char *>(ptr));
+        sample_with(sv, uniforms, shots, mark_bits, out, static_cast<char *>(ptr));
         sv->last_sample_seconds = now() - t0;
     });
 }

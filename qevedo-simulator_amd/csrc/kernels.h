@@ -23,6 +23,12 @@ struct SampleScratch {
     double *targets = nullptr;           // shots, ascending
     void *sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
+    // The walk's running sum every 2^mark_bits amplitudes (6 <= mark_bits,
+    // 2^mark_bits <= the chunk length): ceil(len / 2^mark_bits) values.
+    uint32_t mark_bits = 6;
+    double *marks = nullptr;
+    uint32_t *walked = nullptr;        // per chunk: marks written
+    uint64_t *last_nonzero = nullptr;  // per chunk
 };
 
 // The fixed device workspace of the allocation-free sampling entry point

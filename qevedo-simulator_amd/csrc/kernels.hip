@@ -707,59 +707,138 @@ __global__ void k_bounds(const double *__restrict__ prefix, uint64_t num_chunks,
     bounds[c] = lo;
 }
 
+// The walk (src/measure.rs:143-166) in two kernels. Shot s of chunk c goes
+// to the first index i of the chunk whose running sum prefix[c] + p_start +
+// ... + p_i, added one by one in index order, exceeds targets[s]; a shot no
+// index of the chunk reaches (rounding) goes to the chunk's last nonzero
+// probability. Zero probabilities are never such an index: adding +0.0
+// leaves the sum (never -0.0) unchanged, and every target of the chunk is at
+// least prefix[c]. So the running sum may add them too.
+//
+// k_walk_marks runs each chunk's running sum, one lane per chunk, with no
+// per-shot work, and stores it every 2^mark_bits amplitudes (marks[b] = the
+// sum up to the end of the chunk's block b). It stops after the block where
+// the sum passes the chunk's largest target, and stores how many blocks it
+// walked and the chunk's last nonzero index. k_walk_place then gives every
+// shot a thread of its own: the first mark above its target names its
+// block, and the sum resumes from the mark before (bit for bit the value the
+// sequential sum had there) over at most 2^mark_bits amplitudes.
 template <typename T2>
-__global__ __launch_bounds__(64) void k_walk(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
-                                              uint64_t num_chunks, const double *__restrict__ prefix,
-                                              const uint64_t *__restrict__ bounds, const double *__restrict__ targets,
-                                              uint64_t shots, uint64_t *__restrict__ out) {
+__global__ __launch_bounds__(64) void k_walk_marks(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
+                                                    uint64_t num_chunks, uint32_t mark_bits,
+                                                    const double *__restrict__ prefix,
+                                                    const uint64_t *__restrict__ bounds,
+                                                    const double *__restrict__ targets, uint64_t shots,
+                                                    double *__restrict__ marks, uint32_t *__restrict__ walked,
+                                                    uint64_t *__restrict__ last_nonzero) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= num_chunks) return;
     const uint64_t lo = bounds[c];
     const uint64_t hi = bounds[c + 1] < shots ? bounds[c + 1] : shots;
-    if (lo >= hi) return;
-    double cumulative = prefix[c];
-    uint64_t shot = lo;
+    if (lo >= hi) return;  // no shot in this chunk: nothing is read
+    const double t_last = targets[hi - 1];
     const uint64_t start = c * chunk;
     const uint64_t end = start + chunk < len ? start + chunk : len;
-    uint64_t last_nonzero = start;
-    double next = targets[shot];
-    // Amplitudes are loaded U at a time (the loads of a batch fly together),
-    // then walked one by one in index order. (Two alternating batches of 32,
-    // one in flight while the other is walked, measured slower: 5.3 ms
-    // against 3.6 ms for one batch of 32 and 3.3 ms for one of 64, 10^6
-    // shots at 30 qubits.)
+    const uint64_t block = uint64_t(1) << mark_bits;
+    double *m = marks + (start >> mark_bits);
+    double cumulative = prefix[c];
+    uint32_t last = 0;  // offset from start of the last nonzero probability
+    uint32_t b = 0;
+    // One lane per chunk and few chunks (2^15 at 30 qubits): the loads of a
+    // batch of U amplitudes fly together, then the sum runs over them.
     constexpr int U = 64;
-    uint64_t i = start;
-    for (; i + U <= end && shot < hi; i += U) {
-        T2 v[U];
+    for (uint64_t i = start; i < end;) {
+        const uint64_t stop = i + block < end ? i + block : end;
+        for (; i + U <= stop; i += U) {
+            T2 v[U];
 #pragma unroll
-        for (int j = 0; j < U; j++) v[j] = s[i + j];
+            for (int j = 0; j < U; j++) v[j] = s[i + j];
+            const uint32_t off = (uint32_t)(i - start);
 #pragma unroll
-        for (int j = 0; j < U; j++) {
-            const double p = prob(v[j]);
-            if (p == 0.0) continue;
-            last_nonzero = i + j;
-            cumulative += p;
-            while (shot < hi && next < cumulative) {
-                out[shot] = i + j;
-                shot++;
-                if (shot < hi) next = targets[shot];
+            for (int j = 0; j < U; j++) {
+                const double p = prob(v[j]);
+                last = p == 0.0 ? last : off + j;
+                cumulative += p;
             }
         }
-    }
-    for (; i < end && shot < hi; i++) {
-        const double p = prob(s[i]);
-        if (p == 0.0) continue;
-        last_nonzero = i;
-        cumulative += p;
-        while (shot < hi && next < cumulative) {
-            out[shot] = i;
-            shot++;
-            if (shot < hi) next = targets[shot];
+        for (; i < stop; i++) {
+            const double p = prob(s[i]);
+            last = p == 0.0 ? last : (uint32_t)(i - start);
+            cumulative += p;
         }
+        m[b++] = cumulative;
+        if (t_last < cumulative) break;  // every shot of the chunk lands in a walked block
     }
-    for (; shot < hi; shot++) out[shot] = last_nonzero;  // rounding leftovers
-    // indices are < len by construction: the reference's min(len - 1) is a no-op here
+    walked[c] = b;
+    last_nonzero[c] = start + last;
+}
+
+template <typename T2>
+__global__ __launch_bounds__(256) void k_walk_place(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
+                                                     uint64_t num_chunks, uint32_t mark_bits,
+                                                     const double *__restrict__ prefix,
+                                                     const uint64_t *__restrict__ bounds,
+                                                     const double *__restrict__ targets, uint64_t shots,
+                                                     const double *__restrict__ marks,
+                                                     const uint32_t *__restrict__ walked,
+                                                     const uint64_t *__restrict__ last_nonzero,
+                                                     uint64_t *__restrict__ out) {
+    // Shots [bounds[0], bounds[num_chunks]) fall in some chunk: all of them
+    // for uniforms in [0, 1).
+    const uint64_t first = bounds[0];
+    const uint64_t past = bounds[num_chunks] < shots ? bounds[num_chunks] : shots;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t shot = first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; shot < past; shot += step) {
+        // the chunk: bounds[c] <= shot < bounds[c + 1]
+        uint64_t lo = 0, hi = num_chunks;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (bounds[mid] <= shot) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t c = lo;
+        const double t = targets[shot];
+        const double *m = marks + ((c * chunk) >> mark_bits);
+        const uint32_t nb = walked[c];
+        // the first walked block whose mark is not <= t (a NaN sum counts as
+        // past every target, as no comparison with it succeeds)
+        uint32_t a = 0, z = nb;
+        while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (m[mid] <= t) a = mid + 1;
+            else z = mid;
+        }
+        uint64_t found = last_nonzero[c];
+        if (a < nb) {
+            double cumulative = a == 0 ? prefix[c] : m[a - 1];
+            const uint64_t cend = c * chunk + chunk < len ? c * chunk + chunk : len;
+            uint64_t i = c * chunk + ((uint64_t)a << mark_bits);
+            const uint64_t stop = i + (uint64_t(1) << mark_bits) < cend ? i + (uint64_t(1) << mark_bits) : cend;
+            constexpr int U = 8;
+            bool done = false;
+            for (; i + U <= stop && !done; i += U) {
+                T2 v[U];
+#pragma unroll
+                for (int j = 0; j < U; j++) v[j] = s[i + j];
+#pragma unroll
+                for (int j = 0; j < U; j++) {
+                    cumulative += prob(v[j]);
+                    if (!done && t < cumulative) {
+                        found = i + j;
+                        done = true;
+                    }
+                }
+            }
+            for (; i < stop && !done; i++) {
+                cumulative += prob(s[i]);
+                if (t < cumulative) {
+                    found = i;
+                    done = true;
+                }
+            }
+        }
+        out[shot] = found;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -934,8 +1013,10 @@ hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double
                                                      sizeof(double) * 8, st);
     if (e != hipSuccess) return e;
     k_bounds<<<grid_for(nc + 1, 256, 1u << 30), 256, 0, st>>>(w.prefix, nc, w.targets, shots, w.bounds);
-    k_walk<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.prefix, w.bounds, w.targets,
-                                                          shots, out);
+    k_walk_marks<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.mark_bits, w.prefix, w.bounds,
+                                                                w.targets, shots, w.marks, w.walked, w.last_nonzero);
+    k_walk_place<T2><<<grid_for(shots, 256), 256, 0, st>>>(s, len, chunk, nc, w.mark_bits, w.prefix, w.bounds,
+                                                           w.targets, shots, w.marks, w.walked, w.last_nonzero, out);
     return hipGetLastError();
 }
 

@@ -264,6 +264,51 @@ def test_sampling_bit_exact(precision, n):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_sampling_bit_exact_peaked(precision):
+    """A few amplitudes carry almost all the mass: runs of shots land on the
+    same index, back to back, in several chunks (and in the last amplitude
+    of a chunk), so a walk takes many hits in a row without moving on."""
+    _note(f"sampling peaked {precision}")
+    n = 17
+    r = rng(61)
+    a = random_state(r, n) * 1e-3
+    chunk = 1 << (15 if precision == "single" else 14)
+    for k, idx in enumerate([5, chunk - 1, chunk, 2 * chunk + 77, (1 << n) - 1]):
+        a[idx] = 0.4 + 0.1 * k
+    with StateVector(n, precision) as s:
+        s.set_amplitudes(a / np.linalg.norm(a))
+        stored = s.amplitudes()
+        u = r.random(200_000)
+        got = s.sample(u)
+    want = _oracle_samples(stored, precision, u)
+    assert np.array_equal(got, want)
+    assert np.max(np.bincount(want.astype(np.int64))) > 10_000
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_sampling_rounding_leftovers(precision):
+    """measure.rs:167-168: chunk 1 holds probabilities of about 1e-20 each
+    after a chunk of total 1. Its own sum is positive, so a target of
+    exactly 1.0 falls in it, yet a running sum starting at 1.0 never moves:
+    such shots go to the chunk's last nonzero index."""
+    _note(f"sampling leftovers {precision}")
+    n = 17
+    chunk = 1 << (15 if precision == "single" else 14)
+    a = np.zeros(1 << n, dtype=np.complex128)
+    a[3] = 1.0
+    a[chunk:2 * chunk] = 1e-10
+    a[chunk + 100] = 0
+    u = np.array([0.0, 0.3, 1 - 2 ** -52, 1 - 2 ** -53, 0.999999999999])
+    with StateVector(n, precision) as s:
+        s.set_amplitudes(a)
+        stored = s.amplitudes()
+        got = s.sample(u)
+    want = _oracle_samples(stored, precision, u)
+    assert np.array_equal(got, want)
+    assert np.sum(want == 2 * chunk - 1) == 2
+
+
 def test_sampling_after_circuit_matches_oracle():
     _note("sampling after circuit")
     n = 18
