@@ -215,11 +215,13 @@ int qvd_sv_sample(const qvd_sv *sv, const double *uniforms, uint64_t shots, uint
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). The sampler's buffers (chunk sums, prefix, bounds, uniforms,
- * sorted targets, outcomes, sort temporaries) come from one fixed 64 MiB
- * device array of the library, which every GPU it runs on holds until the
- * process ends. Requests of up to about 1.5 * 10^6 shots on states of up to
- * 33 qubits fit; one whose buffers do not fit returns QVD_EINVAL and must use
- * qvd_sv_sample. The array is
+ * targets, outcomes, sort temporaries, the walk's marks) come from one fixed
+ * 64 MiB device array of the library, which every GPU it runs on holds until
+ * the process ends; the walk's marks are spaced as finely as the array
+ * allows (every 64 amplitudes, as in qvd_sv_sample, up to every chunk).
+ * Requests of up to 1.7 * 10^6 shots on states of up to 33 qubits fit; one
+ * whose buffers do not fit returns QVD_EINVAL and must use qvd_sv_sample.
+ * It runs on the handle's stream only. The array is
  * shared by every handle on a GPU and keeps the last call's contents; calls
  * from one process are serialised, so calls that would overlap on one GPU
  * wait for each other. Same outcomes as qvd_sv_sample. */

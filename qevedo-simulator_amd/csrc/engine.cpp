@@ -89,6 +89,8 @@ struct qvd_sv {
     uint64_t len = 0;
     DBuf amps;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;      // sampling: the uniforms' copy and sort (made by the first qvd_sv_sample)
+    hipEvent_t side_done = nullptr;  // recorded after them
     DBuf scratch;    // small reduction buffers
     DBuf sample_ws;  // sampling workspace, grown to the largest call and kept
     DBuf plan_ws;    // a plan's uploaded programs, tables and matrices, grown to the largest plan and kept
@@ -1389,7 +1391,7 @@ size_t carve_sample(const qvd_sv *sv, uint64_t shots, uint32_t mark_bits, char *
         return at;
     };
     const size_t o_sums = carve(nc * sizeof(double)), o_prefix = carve((nc + 1) * sizeof(double));
-    const size_t o_bounds = carve((nc + 1) * sizeof(uint64_t)), o_tu = carve(shots * sizeof(double));
+    const size_t o_bounds = carve((nc + 1) * sizeof(uint64_t));
     const size_t o_ts = carve(shots * sizeof(double)), o_du = carve(shots * sizeof(double));
     const size_t o_out = carve(shots * sizeof(uint64_t)), o_temp = carve(w.sort_temp_bytes);
     const size_t o_marks = carve(nmarks * sizeof(double)), o_walked = carve(nc * sizeof(uint32_t));
@@ -1401,7 +1403,6 @@ size_t carve_sample(const qvd_sv *sv, uint64_t shots, uint32_t mark_bits, char *
         w.sums = reinterpret_cast<double *>(base + o_sums);
         w.prefix = reinterpret_cast<double *>(base + o_prefix);
         w.bounds = reinterpret_cast<uint64_t *>(base + o_bounds);
-        w.targets_unsorted = reinterpret_cast<double *>(base + o_tu);
         w.targets = reinterpret_cast<double *>(base + o_ts);
         w.sort_temp = base + o_temp;
         *du = reinterpret_cast<double *>(base + o_du);
@@ -1412,16 +1413,31 @@ size_t carve_sample(const qvd_sv *sv, uint64_t shots, uint32_t mark_bits, char *
 
 // Two-pass sampling with the buffers carved from `base` (at least
 // carve_sample bytes of device memory): copies the uniforms in, runs the
-// sampler on the handle's stream, copies the outcomes out and waits.
-void sample_with(qvd_sv *sv, const double *uniforms, uint64_t shots, uint32_t mark_bits, uint64_t *out, char *base) {
+// sampler on the handle's stream, copies the outcomes out and waits. With
+// `side` (a second stream and an event of the handle), the uniforms' copy and
+// sort run there while the chunk sums and their prefix run on the handle's
+// stream; they depend on nothing of each other until the targets are scaled.
+void sample_with(qvd_sv *sv, const double *uniforms, uint64_t shots, uint32_t mark_bits, uint64_t *out, char *base,
+                 bool side) {
     const uint64_t chunk = sample_chunk(sv);
     SampleScratch w;
     double *du = nullptr;
     uint64_t *dout = nullptr;
     carve_sample(sv, shots, mark_bits, base, w, &du, &dout);
-    QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
     with_type(sv, [&](auto *s) {
-        QVD_HIP(launch_sample(s, sv->len, chunk, du, shots, w, dout, sv->stream));
+        if (side) {
+            // the sums first: the copy from pageable memory may hold the host
+            launch_sample_sums(s, sv->len, chunk, w, sv->stream);
+            QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->side));
+            QVD_HIP(launch_sort_uniforms(du, shots, w, sv->side));
+            QVD_HIP(hipEventRecord(sv->side_done, sv->side));
+            QVD_HIP(hipStreamWaitEvent(sv->stream, sv->side_done, 0));
+        } else {
+            QVD_HIP(hipMemcpyAsync(du, uniforms, shots * sizeof(double), hipMemcpyHostToDevice, sv->stream));
+            launch_sample_sums(s, sv->len, chunk, w, sv->stream);
+            QVD_HIP(launch_sort_uniforms(du, shots, w, sv->stream));
+        }
+        QVD_HIP(launch_sample_walk(s, sv->len, chunk, shots, w, dout, sv->stream));
         return 0;
     });
     QVD_HIP(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, sv->stream));
@@ -1504,6 +1520,9 @@ int qvd_sv_destroy(qvd_sv *sv) {
         if (sv->stream) (void)hipStreamSynchronize(sv->stream);
         sv->amps.alloc(0);
         sv->scratch.alloc(0);
+        if (sv->side) (void)hipStreamSynchronize(sv->side);
+        if (sv->side_done) (void)hipEventDestroy(sv->side_done);
+        if (sv->side) (void)hipStreamDestroy(sv->side);
         if (sv->stream) (void)hipStreamDestroy(sv->stream);
         delete sv;
     });
@@ -1874,7 +1893,11 @@ int qvd_sv_sample(const qvd_sv *sv_c, const double *uniforms, uint64_t shots, ui
         SampleScratch w;
         const size_t need = carve_sample(sv, shots, SAMPLE_MARK_BITS, nullptr, w, nullptr, nullptr);
         if (sv->sample_ws.bytes < need) sv->sample_ws.alloc(need);
-        sample_with(sv, uniforms, shots, SAMPLE_MARK_BITS, out, sv->sample_ws.as<char>());
+        if (!sv->side) {
+            QVD_HIP(hipStreamCreateWithFlags(&sv->side, hipStreamNonBlocking));
+            QVD_HIP(hipEventCreateWithFlags(&sv->side_done, hipEventDisableTiming));
+        }
+        sample_with(sv, uniforms, shots, SAMPLE_MARK_BITS, out, sv->sample_ws.as<char>(), true);
         sv->last_sample_seconds = now() - t0;
     });
 }
@@ -1905,7 +1928,7 @@ int qvd_sv_sample_noalloc(const qvd_sv *sv_c, const double *uniforms, uint64_t s
         // process are serialised here (the call returns after its GPU work).
         static std::mutex mu;
         std::lock_guard<std::mutex> lock(mu);
-        sample_with(sv, uniforms, shots, mark_bits, out, static_cast<char *>(ptr));
+        sample_with(sv, uniforms, shots, mark_bits, out, static_cast<char *>(ptr), false);
         sv->last_sample_seconds = now() - t0;
     });
 }

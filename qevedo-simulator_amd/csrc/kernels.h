@@ -19,8 +19,7 @@ struct SampleScratch {
     double *sums = nullptr;              // per chunk
     double *prefix = nullptr;            // chunks + 1
     uint64_t *bounds = nullptr;          // chunks + 1
-    double *targets_unsorted = nullptr;  // shots
-    double *targets = nullptr;           // shots, ascending
+    double *targets = nullptr;           // shots: the sorted uniforms, then the targets, ascending
     void *sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     // The walk's running sum every 2^mark_bits amplitudes (6 <= mark_bits,
@@ -89,8 +88,14 @@ void launch_batch_init(T2 *s, uint64_t len, const uint64_t *offs, uint32_t count
 template <typename T2>
 void launch_collapse(T2 *s, uint64_t len, uint32_t qubit, uint32_t outcome, double keep, hipStream_t st);
 size_t sort_temp_bytes(uint64_t shots);
+// Two-pass sampling (src/measure.rs:104-174) in three parts: the chunk sums
+// and their prefix; the uniforms' sort into w.targets (on any stream); then,
+// after both, targets = sorted uniforms * total, the bounds and the walk.
 template <typename T2>
-hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double *uniforms, uint64_t shots,
-                   SampleScratch &w, uint64_t *out, hipStream_t st);
+void launch_sample_sums(const T2 *s, uint64_t len, uint64_t chunk, SampleScratch &w, hipStream_t st);
+hipError_t launch_sort_uniforms(const double *uniforms, uint64_t shots, SampleScratch &w, hipStream_t st);
+template <typename T2>
+hipError_t launch_sample_walk(const T2 *s, uint64_t len, uint64_t chunk, uint64_t shots, SampleScratch &w,
+                              uint64_t *out, hipStream_t st);
 
 }  // namespace qvd

@@ -642,54 +642,90 @@ __global__ __launch_bounds__(64) void k_chunk_sums(const T2 *__restrict__ s, uin
     sums[c] = acc;
 }
 
-// The sequential prefix over chunk sums, in one lane. Two batches of U sums
-// alternate: one batch's loads fly while the other's adds run, each batch's
-// prefixes overwrite its own registers and are stored after its adds, so the
-// dependent add chain waits neither on memory nor on stores.
-__global__ void k_prefix(const double *__restrict__ sums, uint64_t num_chunks, double *__restrict__ prefix) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        constexpr int U = 32;
-        double p = 0;
-        prefix[0] = 0;
-        uint64_t c = 0;
-        double a[U], b[U];
-        auto load = [&](double(&dst)[U], uint64_t at) {
+// The sequential prefix over chunk sums (src/measure.rs:120-124): one lane
+// adds them one by one. The sums pass through LDS in tiles: while lane 0
+// adds up tile k in place, the workgroup's other lanes store tile k-1's
+// prefixes and load tile k+1, all coalesced, so the dependent add chain
+// waits on LDS only, never on a memory round trip.
+constexpr uint32_t PREFIX_THREADS = 256;
+constexpr uint32_t PREFIX_TILE = 2048;  // doubles per tile; two tiles of LDS
+
+__global__ __launch_bounds__(PREFIX_THREADS) void k_prefix(const double *__restrict__ sums, uint64_t num_chunks,
+                                                          double *__restrict__ prefix) {
+    __shared__ double buf[2][PREFIX_TILE];
+    constexpr uint32_t PER = PREFIX_TILE / PREFIX_THREADS;
+    const uint32_t t = threadIdx.x;
+    const uint64_t tiles = (num_chunks + PREFIX_TILE - 1) / PREFIX_TILE;
+    if (t == 0) prefix[0] = 0;
 #pragma unroll
-            for (int j = 0; j < U; j++) dst[j] = sums[at + j];
-        };
-        auto scan_store = [&](double(&buf)[U], uint64_t at) {
+    for (uint32_t j = 0; j < PER; j++) {
+        const uint64_t c = j * PREFIX_THREADS + t;
+        buf[0][j * PREFIX_THREADS + t] = c < num_chunks ? sums[c] : 0.0;
+    }
+    __syncthreads();
+    double p = 0;
+    for (uint64_t k = 0; k < tiles; k++) {
+        double *cur = buf[k & 1], *other = buf[(k + 1) & 1];
+        const uint64_t base = k * PREFIX_TILE;
+        if (k > 0) {  // tile k-1's prefixes, finished in the last round
 #pragma unroll
-            for (int j = 0; j < U; j++) {
-                p = p + buf[j];
-                buf[j] = p;
+            for (uint32_t j = 0; j < PER; j++) {
+                const uint64_t c = base - PREFIX_TILE + j * PREFIX_THREADS + t;
+                prefix[c + 1] = other[j * PREFIX_THREADS + t];
             }
+        }
+        const bool more = k + 1 < tiles;
+        double v[PER];
+        if (more) {
 #pragma unroll
-            for (int j = 0; j < U; j++) prefix[at + j + 1] = buf[j];
-        };
-        if (c + U <= num_chunks) load(a, c);
-        for (; c + 2 * U <= num_chunks; c += 2 * U) {
-            load(b, c + U);
-            scan_store(a, c);
-            if (c + 3 * U <= num_chunks) load(a, c + 2 * U);
-            scan_store(b, c + U);
+            for (uint32_t j = 0; j < PER; j++) {
+                const uint64_t c = base + PREFIX_TILE + j * PREFIX_THREADS + t;
+                v[j] = c < num_chunks ? sums[c] : 0.0;
+            }
         }
-        if (c + U <= num_chunks) {  // a holds the last whole batch
-            scan_store(a, c);
-            c += U;
+        if (t == 0) {
+            const uint32_t n = num_chunks - base < PREFIX_TILE ? (uint32_t)(num_chunks - base) : PREFIX_TILE;
+            uint32_t i = 0;
+            for (; i + 16 <= n; i += 16) {
+                double x[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) x[j] = cur[i + j];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    p = p + x[j];
+                    x[j] = p;
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) cur[i + j] = x[j];
+            }
+            for (; i < n; i++) {
+                p = p + cur[i];
+                cur[i] = p;
+            }
         }
-        for (; c < num_chunks; c++) {
-            p = p + sums[c];
-            prefix[c + 1] = p;
+        if (more) {  // the same slots this thread read above
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) other[j * PREFIX_THREADS + t] = v[j];
         }
+        __syncthreads();
+    }
+    const double *last = buf[(tiles - 1) & 1];
+    const uint64_t base = (tiles - 1) * PREFIX_TILE;
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) {
+        const uint64_t c = base + j * PREFIX_THREADS + t;
+        if (c < num_chunks) prefix[c + 1] = last[j * PREFIX_THREADS + t];
     }
 }
 
-__global__ void k_targets(const double *__restrict__ u, const double *__restrict__ total_ptr, uint64_t shots,
-                          double *__restrict__ targets) {
+// targets = sorted uniforms * total, in place. The order survives: for a
+// finite total >= 0, rounding u * total is monotone in u, so scaling the
+// sorted uniforms gives sort(u * total) exactly.
+__global__ void k_targets(double *__restrict__ targets, const double *__restrict__ total_ptr, uint64_t shots) {
     const double total = *total_ptr;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < shots; i += stride)
-        targets[i] = u[i] * total;
+        targets[i] = targets[i] * total;
 }
 
 // bounds[c] = number of targets < prefix[c] (partition_point).
@@ -1001,17 +1037,24 @@ size_t sort_temp_bytes(uint64_t shots) {
 }
 
 template <typename T2>
-hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double *uniforms, uint64_t shots,
-                         SampleScratch &w, uint64_t *out, hipStream_t st) {
+void launch_sample_sums(const T2 *s, uint64_t len, uint64_t chunk, SampleScratch &w, hipStream_t st) {
     const uint64_t nc = (len + chunk - 1) / chunk;
     // 64-lane workgroups for the per-chunk kernels, so 2^15 chunks spread over every CU.
     k_chunk_sums<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.sums);
-    k_prefix<<<1, 64, 0, st>>>(w.sums, nc, w.prefix);
-    k_targets<<<grid_for(shots, 256), 256, 0, st>>>(uniforms, w.prefix + nc, shots, w.targets_unsorted);
+    k_prefix<<<1, PREFIX_THREADS, 0, st>>>(w.sums, nc, w.prefix);
+}
+
+hipError_t launch_sort_uniforms(const double *uniforms, uint64_t shots, SampleScratch &w, hipStream_t st) {
     size_t bytes = w.sort_temp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortKeys(w.sort_temp, bytes, w.targets_unsorted, w.targets, (int)shots, 0,
-                                                     sizeof(double) * 8, st);
-    if (e != hipSuccess) return e;
+    return hipcub::DeviceRadixSort::SortKeys(w.sort_temp, bytes, uniforms, w.targets, (int)shots, 0,
+                                             sizeof(double) * 8, st);
+}
+
+template <typename T2>
+hipError_t launch_sample_walk(const T2 *s, uint64_t len, uint64_t chunk, uint64_t shots, SampleScratch &w,
+                              uint64_t *out, hipStream_t st) {
+    const uint64_t nc = (len + chunk - 1) / chunk;
+    k_targets<<<grid_for(shots, 256), 256, 0, st>>>(w.targets, w.prefix + nc, shots);
     k_bounds<<<grid_for(nc + 1, 256, 1u << 30), 256, 0, st>>>(w.prefix, nc, w.targets, shots, w.bounds);
     k_walk_marks<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.mark_bits, w.prefix, w.bounds,
                                                                 w.targets, shots, w.marks, w.walked, w.last_nonzero);
@@ -1040,8 +1083,9 @@ hipError_t launch_sample(const T2 *s, uint64_t len, uint64_t chunk, const double
                                           const uint32_t *, uint32_t, double *, double *, double *, hipStream_t); \
     template void launch_batch_init<T2>(T2 *, uint64_t, const uint64_t *, uint32_t, hipStream_t);              \
     template void launch_collapse<T2>(T2 *, uint64_t, uint32_t, uint32_t, double, hipStream_t);                 \
-    template hipError_t launch_sample<T2>(const T2 *, uint64_t, uint64_t, const double *, uint64_t,             \
-                                          SampleScratch &, uint64_t *, hipStream_t);
+    template void launch_sample_sums<T2>(const T2 *, uint64_t, uint64_t, SampleScratch &, hipStream_t);         \
+    template hipError_t launch_sample_walk<T2>(const T2 *, uint64_t, uint64_t, uint64_t, SampleScratch &,       \
+                                               uint64_t *, hipStream_t);
 
 QVD_INSTANTIATE(float2)
 QVD_INSTANTIATE(double2)

@@ -309,6 +309,19 @@ def test_sampling_rounding_leftovers(precision):
     assert np.sum(want == 2 * chunk - 1) == 2
 
 
+def test_sampling_bit_exact_many_chunks():
+    """27 qubits in complex64: 4096 chunks, so the prefix over chunk sums
+    runs over more than one of the device's staging tiles."""
+    _note("sampling 27 qubits")
+    n = 27
+    with StateVector(n, "single") as s:
+        s.apply_ops(lower_gates(library.random_circuit(n, 3, 12)))
+        u = rng(27).random(50_000)
+        got = s.sample(u)
+        amps = s.amplitudes()
+    assert np.array_equal(got, _oracle_samples(amps, "single", u))
+
+
 def test_sampling_after_circuit_matches_oracle():
     _note("sampling after circuit")
     n = 18
