@@ -619,6 +619,61 @@ __global__ void k_collapse(T2 *__restrict__ s, uint64_t len, uint32_t qubit, uin
 // the prefix over chunks is sequential, and the walk adds probabilities one
 // by one; so one lane owns one chunk. Loads are 16-byte vectors.
 // ---------------------------------------------------------------------------
+// Amplitudes per batch of the one-lane-per-chunk passes: two batches live in
+// registers (128 VGPRs of amplitudes).
+template <typename T2>
+constexpr int chunk_batch() {
+    return sizeof(T2) == 8 ? 32 : 16;
+}
+
+// Streams one lane's amplitudes [i, end) through visit(batch, first index) in
+// batches of U, two at a time in registers: batch b + 1's loads fly while
+// batch b is visited. With one lane per chunk and few chunks (2^15 at 30
+// qubits, half a wave per SIMD) nothing else covers a batch's memory round
+// trip. visit returns false to stop (`stopped` is then set). Returns the
+// first index not visited; fewer than U amplitudes are left from there unless
+// it stopped.
+template <int U, typename T2, typename F>
+__device__ __forceinline__ uint64_t stream_batches(const T2 *__restrict__ s, uint64_t i, uint64_t end, bool &stopped,
+                                                   F &&visit) {
+    stopped = false;
+    if (i + U > end) return i;
+    T2 a[U], b[U];
+    uint64_t a_at = i;  // a holds [a_at, a_at + U)
+#pragma unroll
+    for (int j = 0; j < U; j++) a[j] = s[i + j];
+    while (i + 2 * U <= end) {
+#pragma unroll
+        for (int j = 0; j < U; j++) b[j] = s[i + U + j];
+        __builtin_amdgcn_sched_barrier(0);  // b's loads go out before a is used
+        if (!visit(a, i)) {
+            stopped = true;
+            return i + U;
+        }
+        i += U;
+        // The next batch, or the last whole one (in bounds: i + U <= end):
+        // an unconditional load keeps the waits for b exact.
+        a_at = i + 2 * U <= end ? i + U : end - U;
+#pragma unroll
+        for (int j = 0; j < U; j++) a[j] = s[a_at + j];
+        __builtin_amdgcn_sched_barrier(0);  // a's loads go out before b is used
+        if (!visit(b, i)) {
+            stopped = true;
+            return i + U;
+        }
+        i += U;
+    }
+    if (i + U <= end) {
+        if (a_at != i) {
+#pragma unroll
+            for (int j = 0; j < U; j++) a[j] = s[i + j];
+        }
+        if (!visit(a, i)) stopped = true;
+        i += U;
+    }
+    return i;
+}
+
 template <typename T2>
 __global__ __launch_bounds__(64) void k_chunk_sums(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
                                                     uint64_t num_chunks, double *__restrict__ sums) {
@@ -627,17 +682,13 @@ __global__ __launch_bounds__(64) void k_chunk_sums(const T2 *__restrict__ s, uin
     const uint64_t start = c * chunk;
     const uint64_t end = start + chunk < len ? start + chunk : len;
     double acc = 0;
-    uint64_t i = start;
-    // One lane per chunk and few chunks (2^15 at 30 qubits): each batch of
-    // loads is a memory round trip the lane waits for, so batches are long.
-    constexpr int U = 32;
-    for (; i + U <= end; i += U) {
-        T2 v[U];
-#pragma unroll
-        for (int j = 0; j < U; j++) v[j] = s[i + j];
+    constexpr int U = chunk_batch<T2>();
+    bool stopped;
+    uint64_t i = stream_batches<U>(s, start, end, stopped, [&](const T2(&v)[U], uint64_t) {
 #pragma unroll
         for (int j = 0; j < U; j++) acc += prob(v[j]);
-    }
+        return true;
+    });
     for (; i < end; i++) acc += prob(s[i]);
     sums[c] = acc;
 }
@@ -775,35 +826,38 @@ __global__ __launch_bounds__(64) void k_walk_marks(const T2 *__restrict__ s, uin
     const double t_last = targets[hi - 1];
     const uint64_t start = c * chunk;
     const uint64_t end = start + chunk < len ? start + chunk : len;
-    const uint64_t block = uint64_t(1) << mark_bits;
+    const uint64_t block_mask = (uint64_t(1) << mark_bits) - 1;
     double *m = marks + (start >> mark_bits);
     double cumulative = prefix[c];
     uint32_t last = 0;  // offset from start of the last nonzero probability
     uint32_t b = 0;
-    // One lane per chunk and few chunks (2^15 at 30 qubits): the loads of a
-    // batch of U amplitudes fly together, then the sum runs over them.
-    constexpr int U = 64;
-    for (uint64_t i = start; i < end;) {
-        const uint64_t stop = i + block < end ? i + block : end;
-        for (; i + U <= stop; i += U) {
-            T2 v[U];
+    bool pending = false;  // amplitudes added since the last mark
+    constexpr int U = chunk_batch<T2>();  // divides 2^mark_bits
+    bool stopped;
+    uint64_t i = stream_batches<U>(s, start, end, stopped, [&](const T2(&v)[U], uint64_t at) {
+        const uint32_t off = (uint32_t)(at - start);
 #pragma unroll
-            for (int j = 0; j < U; j++) v[j] = s[i + j];
-            const uint32_t off = (uint32_t)(i - start);
-#pragma unroll
-            for (int j = 0; j < U; j++) {
-                const double p = prob(v[j]);
-                last = p == 0.0 ? last : off + j;
-                cumulative += p;
-            }
+        for (int j = 0; j < U; j++) {
+            const double p = prob(v[j]);
+            last = p == 0.0 ? last : off + j;
+            cumulative += p;
         }
-        for (; i < stop; i++) {
+        if (((at + U - start) & block_mask) == 0 || at + U == end) {
+            m[b++] = cumulative;
+            pending = false;
+            return !(t_last < cumulative);  // every shot of the chunk lands in a walked block
+        }
+        pending = true;
+        return true;
+    });
+    if (!stopped) {  // fewer than U amplitudes left: a state of fewer than U amplitudes
+        for (; i < end; i++) {
             const double p = prob(s[i]);
             last = p == 0.0 ? last : (uint32_t)(i - start);
             cumulative += p;
+            pending = true;
         }
-        m[b++] = cumulative;
-        if (t_last < cumulative) break;  // every shot of the chunk lands in a walked block
+        if (pending) m[b++] = cumulative;
     }
     walked[c] = b;
     last_nonzero[c] = start + last;
