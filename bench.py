@@ -398,19 +398,28 @@ def main() -> int:
         sv.scale(1.0)
 
     # Sampling: 10^6 shots through the two-pass sampler; the first call also
-    # sizes the handle's sampling workspace, the second is the steady state.
+    # sizes the handle's sampling workspace and second stream. The steady
+    # state samples into an output array already in use (best of 3); a new
+    # output array adds the host's first-touch page faults to the copy out.
     sv.set_zero()
     sv.apply_ops(ops, **kw)
     if args.shots:
         u = np.random.Generator(np.random.PCG64(7)).random(args.shots)
         first = sv.sample(u)
         first_s = sv.last_sample_seconds()
-        again = sv.sample(u)
+        fresh = sv.sample(u)
+        fresh_s = sv.last_sample_seconds()
+        steady = []
+        for _ in range(3):
+            again = sv.sample(u, out=fresh)
+            steady.append(sv.last_sample_seconds())
         assert np.array_equal(first, again)
-        result_extra["sample_seconds"] = sv.last_sample_seconds()
+        result_extra["sample_seconds"] = min(steady)
+        result_extra["sample_seconds_new_output_array"] = fresh_s
         result_extra["sample_seconds_first_call"] = first_s
         result_extra["shots"] = args.shots
-        log(f"sampling {args.shots} shots: {result_extra['sample_seconds']:.4f} s (first call {first_s:.4f} s)")
+        log(f"sampling {args.shots} shots: {result_extra['sample_seconds']:.4f} s (into a new output array "
+            f"{fresh_s:.4f} s; first call {first_s:.4f} s)")
 
     error = None
     if not args.no_check:

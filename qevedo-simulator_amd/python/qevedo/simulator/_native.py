@@ -438,9 +438,18 @@ class StateVector:
     def reset(self, qubit: int, uniform: float) -> None:
         _check(lib().qvd_sv_reset(self.h, qubit, uniform))
 
-    def sample(self, uniforms) -> np.ndarray:
+    def sample(self, uniforms, out=None) -> np.ndarray:
+        """Outcomes for the uniforms (sample_unordered, measure.rs:104-174).
+        `out`, a C-contiguous uint64 array of one element per uniform, is
+        filled and returned instead of a new array: a caller sampling
+        repeatedly saves the host's first-touch page faults of a new 8-byte-
+        per-shot array, which the copy of the outcomes would otherwise pay."""
         u = np.ascontiguousarray(uniforms, dtype=np.float64)
-        out = np.empty(u.size, dtype=np.uint64)
+        if out is None:
+            out = np.empty(u.size, dtype=np.uint64)
+        elif (not isinstance(out, np.ndarray) or out.dtype != np.uint64 or out.size != u.size
+              or not out.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError("out must be a writeable C-contiguous uint64 array with one element per uniform")
         _check(lib().qvd_sv_sample(self.h, _p(u, C.c_double), u.size, _p(out, C.c_uint64)))
         return out
 

@@ -334,6 +334,22 @@ def test_sampling_after_circuit_matches_oracle():
     assert np.array_equal(got, _oracle_samples(amps, "single", u))
 
 
+def test_sampling_into_a_given_array():
+    _note("sampling into out")
+    n = 16
+    with StateVector(n, "single") as s:
+        s.apply_ops(lower_gates(library.random_circuit(n, 6, 2)))
+        u = rng(3).random(30_000)
+        want = s.sample(u)
+        buf = np.zeros(u.size, dtype=np.uint64)
+        assert s.sample(u, out=buf) is buf
+        assert np.array_equal(buf, want)
+        for bad in [np.zeros(u.size, dtype=np.int64), np.zeros(u.size - 1, dtype=np.uint64),
+                    np.zeros((u.size, 2), dtype=np.uint64)[:, 0]]:
+            with pytest.raises(ValueError):
+                s.sample(u, out=bad)
+
+
 def test_bell_and_distribution_sampling():
     # tests/simulator.rs:59-111
     _note("bell sampling")
