@@ -26,8 +26,7 @@ struct SampleScratch {
     // 2^mark_bits <= the chunk length): ceil(len / 2^mark_bits) values.
     uint32_t mark_bits = 6;
     double *marks = nullptr;
-    uint32_t *walked = nullptr;        // per chunk: marks written
-    uint64_t *last_nonzero = nullptr;  // per chunk
+    uint32_t *walked = nullptr;  // per chunk: marks written
 };
 
 // The fixed device workspace of the allocation-free sampling entry point

@@ -630,12 +630,14 @@ constexpr int chunk_batch() {
 // batches of U, two at a time in registers: batch b + 1's loads fly while
 // batch b is visited. With one lane per chunk and few chunks (2^15 at 30
 // qubits, half a wave per SIMD) nothing else covers a batch's memory round
-// trip. visit returns false to stop (`stopped` is then set). Returns the
-// first index not visited; fewer than U amplitudes are left from there unless
-// it stopped.
-template <int U, typename T2, typename F>
+// trip. visit returns false to stop (`stopped` is then set). issued() runs
+// right after each batch's loads go out: a store placed there follows the
+// loads the next wait is for, so that wait does not include the store (the
+// memory counter of loads and stores runs in order). Returns the first index
+// not visited; fewer than U amplitudes are left from there unless it stopped.
+template <int U, typename T2, typename F, typename G>
 __device__ __forceinline__ uint64_t stream_batches(const T2 *__restrict__ s, uint64_t i, uint64_t end, bool &stopped,
-                                                   F &&visit) {
+                                                   F &&visit, G &&issued) {
     stopped = false;
     if (i + U > end) return i;
     T2 a[U], b[U];
@@ -646,6 +648,7 @@ __device__ __forceinline__ uint64_t stream_batches(const T2 *__restrict__ s, uin
 #pragma unroll
         for (int j = 0; j < U; j++) b[j] = s[i + U + j];
         __builtin_amdgcn_sched_barrier(0);  // b's loads go out before a is used
+        issued();
         if (!visit(a, i)) {
             stopped = true;
             return i + U;
@@ -657,6 +660,7 @@ __device__ __forceinline__ uint64_t stream_batches(const T2 *__restrict__ s, uin
 #pragma unroll
         for (int j = 0; j < U; j++) a[j] = s[a_at + j];
         __builtin_amdgcn_sched_barrier(0);  // a's loads go out before b is used
+        issued();
         if (!visit(b, i)) {
             stopped = true;
             return i + U;
@@ -668,6 +672,7 @@ __device__ __forceinline__ uint64_t stream_batches(const T2 *__restrict__ s, uin
 #pragma unroll
             for (int j = 0; j < U; j++) a[j] = s[i + j];
         }
+        issued();  // between every two visits
         if (!visit(a, i)) stopped = true;
         i += U;
     }
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(64) void k_chunk_sums(const T2 *__restrict__ s, uin
 #pragma unroll
         for (int j = 0; j < U; j++) acc += prob(v[j]);
         return true;
-    });
+    }, [] {});
     for (; i < end; i++) acc += prob(s[i]);
     sums[c] = acc;
 }
@@ -806,18 +811,19 @@ __global__ void k_bounds(const double *__restrict__ prefix, uint64_t num_chunks,
 // per-shot work, and stores it every 2^mark_bits amplitudes (marks[b] = the
 // sum up to the end of the chunk's block b). It stops after the block where
 // the sum passes the chunk's largest target, and stores how many blocks it
-// walked and the chunk's last nonzero index. k_walk_place then gives every
-// shot a thread of its own: the first mark above its target names its
-// block, and the sum resumes from the mark before (bit for bit the value the
-// sequential sum had there) over at most 2^mark_bits amplitudes.
+// walked. k_walk_place then gives every shot a thread of its own: the first
+// mark above its target names its block, and the sum resumes from the mark
+// before (bit for bit the value the sequential sum had there) over at most
+// 2^mark_bits amplitudes. A shot that no index reaches was left over by a
+// walk that went to the chunk's end; its thread scans back from the end for
+// the last nonzero probability (rare, and cut short by the first nonzero).
 template <typename T2>
 __global__ __launch_bounds__(64) void k_walk_marks(const T2 *__restrict__ s, uint64_t len, uint64_t chunk,
                                                     uint64_t num_chunks, uint32_t mark_bits,
                                                     const double *__restrict__ prefix,
                                                     const uint64_t *__restrict__ bounds,
                                                     const double *__restrict__ targets, uint64_t shots,
-                                                    double *__restrict__ marks, uint32_t *__restrict__ walked,
-                                                    uint64_t *__restrict__ last_nonzero) {
+                                                    double *__restrict__ marks, uint32_t *__restrict__ walked) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= num_chunks) return;
     const uint64_t lo = bounds[c];
@@ -829,38 +835,61 @@ __global__ __launch_bounds__(64) void k_walk_marks(const T2 *__restrict__ s, uin
     const uint64_t block_mask = (uint64_t(1) << mark_bits) - 1;
     double *m = marks + (start >> mark_bits);
     double cumulative = prefix[c];
-    uint32_t last = 0;  // offset from start of the last nonzero probability
     uint32_t b = 0;
     bool pending = false;  // amplitudes added since the last mark
+    // A mark is stored after the next batch's loads go out (see
+    // stream_batches): a store before them would make the wait for them wait
+    // for the store too, a memory round trip per mark. Marks gather in LDS
+    // and leave 16 at a time, a whole 128-byte line per lane: one 8-byte
+    // store per mark left lines partly written while the chunk streamed
+    // through the caches.
+    constexpr uint32_t MG = 16;
+    __shared__ double mbuf[64][MG + 1];
+    const uint32_t lane = threadIdx.x;
+    uint32_t nbuf = 0, flushed = 0;
+    bool mark_ready = false;
+    double mark = 0;
+    auto store_mark = [&] {
+        if (mark_ready) {
+            mbuf[lane][nbuf++] = mark;
+            b++;
+            if (nbuf == MG) {
+#pragma unroll
+                for (uint32_t k = 0; k < MG; k++) m[flushed + k] = mbuf[lane][k];
+                flushed += MG;
+                nbuf = 0;
+            }
+        }
+        mark_ready = false;
+    };
     constexpr int U = chunk_batch<T2>();  // divides 2^mark_bits
     bool stopped;
     uint64_t i = stream_batches<U>(s, start, end, stopped, [&](const T2(&v)[U], uint64_t at) {
-        const uint32_t off = (uint32_t)(at - start);
 #pragma unroll
-        for (int j = 0; j < U; j++) {
-            const double p = prob(v[j]);
-            last = p == 0.0 ? last : off + j;
-            cumulative += p;
-        }
+        for (int j = 0; j < U; j++) cumulative += prob(v[j]);
         if (((at + U - start) & block_mask) == 0 || at + U == end) {
-            m[b++] = cumulative;
+            mark = cumulative;
+            mark_ready = true;
             pending = false;
             return !(t_last < cumulative);  // every shot of the chunk lands in a walked block
         }
         pending = true;
         return true;
-    });
+    }, store_mark);
+    store_mark();
     if (!stopped) {  // fewer than U amplitudes left: a state of fewer than U amplitudes
         for (; i < end; i++) {
-            const double p = prob(s[i]);
-            last = p == 0.0 ? last : (uint32_t)(i - start);
-            cumulative += p;
+            cumulative += prob(s[i]);
             pending = true;
         }
-        if (pending) m[b++] = cumulative;
+        if (pending) {
+            mark = cumulative;
+            mark_ready = true;
+            store_mark();
+        }
     }
+    for (uint32_t k = 0; k < nbuf; k++) m[flushed + k] = mbuf[lane][k];
     walked[c] = b;
-    last_nonzero[c] = start + last;
 }
 
 template <typename T2>
@@ -871,7 +900,6 @@ __global__ __launch_bounds__(256) void k_walk_place(const T2 *__restrict__ s, ui
                                                      const double *__restrict__ targets, uint64_t shots,
                                                      const double *__restrict__ marks,
                                                      const uint32_t *__restrict__ walked,
-                                                     const uint64_t *__restrict__ last_nonzero,
                                                      uint64_t *__restrict__ out) {
     // Shots [bounds[0], bounds[num_chunks]) fall in some chunk: all of them
     // for uniforms in [0, 1).
@@ -898,14 +926,14 @@ __global__ __launch_bounds__(256) void k_walk_place(const T2 *__restrict__ s, ui
             if (m[mid] <= t) a = mid + 1;
             else z = mid;
         }
-        uint64_t found = last_nonzero[c];
+        const uint64_t cend = c * chunk + chunk < len ? c * chunk + chunk : len;
+        uint64_t found = 0;
+        bool done = false;
         if (a < nb) {
             double cumulative = a == 0 ? prefix[c] : m[a - 1];
-            const uint64_t cend = c * chunk + chunk < len ? c * chunk + chunk : len;
             uint64_t i = c * chunk + ((uint64_t)a << mark_bits);
             const uint64_t stop = i + (uint64_t(1) << mark_bits) < cend ? i + (uint64_t(1) << mark_bits) : cend;
             constexpr int U = 8;
-            bool done = false;
             for (; i + U <= stop && !done; i += U) {
                 T2 v[U];
 #pragma unroll
@@ -926,6 +954,14 @@ __global__ __launch_bounds__(256) void k_walk_place(const T2 *__restrict__ s, ui
                     done = true;
                 }
             }
+        }
+        if (!done) {  // a rounding leftover: the chunk's last nonzero probability (a NaN counts), else its start
+            found = c * chunk;
+            for (uint64_t i = cend; i-- > c * chunk;)
+                if (!(prob(s[i]) == 0.0)) {
+                    found = i;
+                    break;
+                }
         }
         out[shot] = found;
     }
@@ -1111,9 +1147,9 @@ hipError_t launch_sample_walk(const T2 *s, uint64_t len, uint64_t chunk, uint64_
     k_targets<<<grid_for(shots, 256), 256, 0, st>>>(w.targets, w.prefix + nc, shots);
     k_bounds<<<grid_for(nc + 1, 256, 1u << 30), 256, 0, st>>>(w.prefix, nc, w.targets, shots, w.bounds);
     k_walk_marks<T2><<<grid_for(nc, 64, 1u << 30), 64, 0, st>>>(s, len, chunk, nc, w.mark_bits, w.prefix, w.bounds,
-                                                                w.targets, shots, w.marks, w.walked, w.last_nonzero);
+                                                                w.targets, shots, w.marks, w.walked);
     k_walk_place<T2><<<grid_for(shots, 256), 256, 0, st>>>(s, len, chunk, nc, w.mark_bits, w.prefix, w.bounds,
-                                                           w.targets, shots, w.marks, w.walked, w.last_nonzero, out);
+                                                           w.targets, shots, w.marks, w.walked, out);
     return hipGetLastError();
 }
 
