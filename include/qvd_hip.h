@@ -75,6 +75,9 @@ typedef struct qvd_options {
     uint32_t flags;            /* QVD_FLAG_* */
     uint32_t variant;          /* heuristic plan variant, 0..QVD_PLAN_VARIANTS-1 (0 = the heuristic);
                                   the learned planners choose it themselves */
+    uint32_t fixed_lanes;      /* register-tile geometry of qvd_sv_apply_ops: the number F of lane bits
+                                  pinned to qubits 0..F-1 (as qvd_circuit.fixed_lanes); 0 = the default, or
+                                  the autotuner's choice with QVD_FLAG_AUTOTUNE. New in this engine. */
 } qvd_options;
 
 /* Plan variants (blocking.rs's lookahead with a reduced capacity, a greedy
@@ -91,12 +94,18 @@ typedef struct qvd_options {
 /* qvd_options.flags: apply 5- and 6-qubit dense ops in complex64 with VALU
  * FMAs instead of the FP32 matrix cores (for comparisons). */
 #define QVD_FLAG_DENSE_VALU 2u
-/* qvd_options.flags for qvd_batch_create: pick each circuit's register-tile
- * geometry (qvd_circuit.fixed_lanes left 0) by timing. Circuits of the same
- * size and gate layout form a class; each class is run as a batch of its own
- * (up to enough circuits to fill the device a few times) with every candidate
- * geometry, and the fastest is kept, for this batch and for later batches of
- * the process with that class. New in this engine. */
+/* qvd_options.flags: pick the register-tile geometry (qvd_circuit.fixed_lanes
+ * or qvd_options.fixed_lanes left 0) by timing. Circuits of the same size and
+ * gate layout form a class, and the fastest geometry is kept for this call
+ * and for later ones of the process with that class. qvd_batch_create runs
+ * each class as a batch of its own (up to enough circuits to fill the device
+ * a few times) with every candidate geometry. qvd_sv_apply_ops runs the
+ * class's plan once with each candidate on a scratch state of the same size
+ * (allocated for the probes and freed; if it cannot be allocated the default
+ * stays), the first time a class comes; the caller's state is untouched by
+ * the probes. The geometry changes which code path a gate takes, so the
+ * amplitudes can differ in the last bits between geometries. New in this
+ * engine. */
 #define QVD_FLAG_AUTOTUNE 4u
 /* qvd_options.flags: visit every register tile. By default a stage skips the
  * tiles known to be zero: a qubit that no non-diagonal op has targeted since
@@ -125,6 +134,7 @@ typedef struct qvd_stats {
     uint64_t other_bytes;         /* algorithmic HBM bytes of the other passes (full-state ops, writing a
                                      pending |0...0>, a batch's initialisation and expectation values) */
     uint64_t skipped_tiles;       /* register tiles not visited because they are known to be zero */
+    uint64_t fixed_lanes;         /* register-tile geometry the stages ran with (qvd_options.fixed_lanes) */
 } qvd_stats;
 
 const char *qvd_last_error(void);

@@ -479,7 +479,8 @@ struct LaunchTimer {
 // Runs a plan on the state. zero: the qubits known to be |0> before it (all
 // of them when the state is a pending |0...0>); returns those after it.
 template <typename T2>
-uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags, uint64_t zero) {
+uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &st, uint32_t flags, uint64_t zero,
+                  uint32_t fixed_lanes) {
     const bool timing = (flags & QVD_FLAG_TIMING) != 0;
     LaunchTimer timer;
     timer.on = timing;
@@ -496,7 +497,8 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
     std::vector<uint32_t> vints;
     std::vector<VStageDesc> vdescs;
     PendingDiag pend(sv->n);
-    const VGeom geom = vstage_geom<T2>();
+    VGeom geom = vstage_geom<T2>();
+    if (fixed_lanes) geom.f_bits = fixed_lanes;
     const bool skip_zero = (flags & QVD_FLAG_ALL_TILES) == 0;
     const uint64_t state_bytes = sv->len * sizeof(T2);
     // A pending |0...0> is made in registers by the first step when that is a
@@ -566,7 +568,7 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
             timer.begin(timer.stage);
             reg_launch.push_back(1);
             QVD_HIP(launch_vstage<T2>(s, dvdescs + vi, 1, nullptr, dvins, dvco, dvlds, dvints, grid, sv->stream,
-                                      d.from_zero != 0));
+                                      d.from_zero != 0, geom.f_bits));
             vi++;
             timer.end(timer.stage);
             st.stages++;
@@ -672,8 +674,9 @@ std::vector<Step> split_for_coefficients(std::vector<Step> steps, uint32_t n, in
 // images give bit-identical amplitudes; plans that apply two gates on
 // different qubits in a different order round differently and have
 // different images.
-std::vector<uint8_t> plan_image(const std::vector<Step> &steps, uint32_t n, int precision) {
-    const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+std::vector<uint8_t> plan_image(const std::vector<Step> &steps, uint32_t n, int precision, uint32_t fixed_lanes = 0) {
+    VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    if (fixed_lanes) vg.f_bits = fixed_lanes;
     std::vector<uint8_t> img;
     auto put = [&](const void *p, size_t bytes) {
         const uint8_t *b = static_cast<const uint8_t *>(p);
@@ -721,8 +724,9 @@ std::vector<uint8_t> plan_image(const std::vector<Step> &steps, uint32_t n, int 
 // Features of each register-tile stage of a plan, in launch order: what the
 // stage's program does per tile (counts of each instruction kind, in
 // log(1 + count)), the state size and the precision.
-std::vector<float> plan_features(const std::vector<Step> &steps, uint32_t n, int precision) {
-    const VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+std::vector<float> plan_features(const std::vector<Step> &steps, uint32_t n, int precision, uint32_t fixed_lanes = 0) {
+    VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    if (fixed_lanes) vg.f_bits = fixed_lanes;
     std::vector<float> rows;
     PendingDiag pend(n);
     std::vector<C64> co;
@@ -807,9 +811,11 @@ uint64_t fnv(uint64_t h, const void *p, size_t bytes) {
     return h;
 }
 
-uint64_t ops_hash(uint32_t n, int precision, const qvd_options &opt, const qvd_op *ops, uint64_t num_ops) {
+uint64_t ops_hash(uint32_t n, int precision, const qvd_options &opt, const qvd_op *ops, uint64_t num_ops,
+                  uint32_t fixed_lanes) {
     uint64_t h = 1469598103934665603ull;
     h = fnv(h, &n, sizeof n);
+    h = fnv(h, &fixed_lanes, sizeof fixed_lanes);
     h = fnv(h, &precision, sizeof precision);
     h = fnv(h, &opt.max_fused_qubits, sizeof opt.max_fused_qubits);
     h = fnv(h, &opt.tile_bits, sizeof opt.tile_bits);
@@ -846,14 +852,14 @@ std::vector<uint32_t> plan_signature(const std::vector<Step> &steps) {
 // The variants whose plans have the same image as variant `base` (itself
 // included): the plans that reproduce its amplitudes bit for bit.
 std::vector<uint32_t> equivalent_variants(const std::vector<std::vector<Step>> &plans, uint32_t base, uint32_t n,
-                                          int precision) {
+                                          int precision, uint32_t fixed_lanes = 0) {
     std::vector<std::vector<uint8_t>> img(plans.size());
     std::vector<std::thread> workers;
     std::vector<std::exception_ptr> errors(plans.size());
     for (size_t var = 0; var < plans.size(); var++)
         workers.emplace_back([&, var] {
             try {
-                img[var] = plan_image(plans[var], n, precision);
+                img[var] = plan_image(plans[var], n, precision, fixed_lanes);
             } catch (...) {
                 errors[var] = std::current_exception();
             }
@@ -887,7 +893,12 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
     if (opt.tile_bits < 0) return plan_unblocked(v, opt.max_fused_qubits);
     TileConfig tc = default_tiles(precision);
     VGeom vg = precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
-    if (fixed_lanes) vg.f_bits = fixed_lanes;  // a batch's per-circuit geometry
+    if (!fixed_lanes) fixed_lanes = opt.fixed_lanes;  // a batch's per-circuit geometry, or the caller's
+    if (fixed_lanes) {
+        require(fixed_lanes + 1 >= vg.f_bits && fixed_lanes <= vg.f_bits + 1,
+                "fixed_lanes must be 0 or " + std::to_string(vg.f_bits - 1) + ".." + std::to_string(vg.f_bits + 1));
+        vg.f_bits = fixed_lanes;
+    }
     PlanParams p;
     p.num_qubits = n;
     p.max_fused = std::max<uint32_t>(opt.max_fused_qubits, 1);
@@ -915,7 +926,7 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
     }
     const bool same_amplitudes = opt.planner == QVD_PLANNER_LEARNED;
     // Learned: a choice made for the same ops before is reused.
-    const uint64_t key = ops_hash(n, precision, opt, ops, num_ops);
+    const uint64_t key = ops_hash(n, precision, opt, ops, num_ops, vg.f_bits);
     auto hit = g_model.chosen.find(key);
     if (hit != g_model.chosen.end()) {
         p.variant = hit->second;
@@ -957,7 +968,7 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
                 if (!dup[u] && sigs[u] == sigs[var]) dup[var] = 1;
         }
         if (same_amplitudes) {
-            std::vector<uint32_t> keep = equivalent_variants(plans, base, n, precision);
+            std::vector<uint32_t> keep = equivalent_variants(plans, base, n, precision, vg.f_bits);
             for (uint32_t var = 0; var < PLAN_VARIANTS; var++)
                 if (std::find(keep.begin(), keep.end(), var) == keep.end()) dup[var] = 1;
         }
@@ -966,7 +977,7 @@ std::vector<Step> make_plan(uint32_t n, int precision, const qvd_op *ops, uint64
             if (dup[var]) continue;
             workers.emplace_back([&, var] {
                 try {
-                    feats[var] = plan_features(plans[var], n, precision);
+                    feats[var] = plan_features(plans[var], n, precision, vg.f_bits);
                 } catch (...) {
                     errors[var] = std::current_exception();
                 }
@@ -1659,28 +1670,106 @@ int qvd_plan_ops(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t
     });
 }
 
+namespace qvd {
+namespace {
+
+// Plans `ops` and runs the plan on `sv` (qvd_sv_apply_ops after the
+// geometry is settled).
+void apply_plan(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options &opt, qvd_stats &st) {
+    const double t0 = now();
+    uint32_t variant = 0;
+    std::vector<Step> steps = make_plan(sv->n, sv->precision, ops, num_ops, opt, &variant);
+    const double t1 = now();
+    st = qvd_stats{};
+    st.gates = num_ops;
+    st.plan_variant = variant;
+    st.fixed_lanes = opt.fixed_lanes ? opt.fixed_lanes
+                                     : (sv->precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>()).f_bits;
+    // An empty plan leaves a pending |0...0> pending.
+    sv->bind();
+    if (!steps.empty()) {
+        const uint64_t zero = sv->zero_pending ? qubit_mask(sv->n) : sv->zero_qubits;
+        sv->zero_qubits = 0;  // until the plan has run
+        sv->zero_qubits = with_type(sv, [&](auto *s) {
+            return run_plan(sv, s, steps, st, opt.flags, zero, opt.fixed_lanes);
+        });
+    }
+    const double t2 = now();
+    st.plan_seconds = t1 - t0;
+    st.gate_seconds = t2 - t0;
+}
+
+// The single-state geometry autotuner's choices for this process.
+TuneCache g_tune_sv;
+
+// The register-tile geometry for `ops` on `sv` under QVD_FLAG_AUTOTUNE: the
+// fastest of the default and one fixed lane bit fewer or more (kept only if
+// 1 % faster than the default), timed once per circuit class by running the
+// plan twice with each on a scratch state of the same size that knows the
+// same qubits to be zero (the amplitudes do not change the time; the tiles
+// known to be zero do). 0 (the default) when the plan has no register-tile
+// stages or the scratch state cannot be allocated.
+uint32_t tune_geometry(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options &opt) {
+    const VGeom vg = sv->precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>();
+    if (opt.tile_bits != 0 || sv->n < vg.L()) return 0;
+    qvd_circuit c{};
+    c.num_qubits = sv->n;
+    c.ops = ops;
+    c.num_ops = num_ops;
+    const uint64_t key = circuit_class(c, sv->precision, opt) ^
+                         (0x9e3779b97f4a7c15ull * (1 + opt.planner + 8 * uint64_t(opt.flags & QVD_FLAG_ALL_TILES)));
+    {
+        std::lock_guard<std::mutex> lock(g_tune_sv.mu);
+        auto hit = g_tune_sv.choice.find(key);
+        if (hit != g_tune_sv.choice.end()) return hit->second;
+    }
+    uint32_t best = 0;
+    qvd_sv *probe = nullptr;
+    if (qvd_sv_create(sv->n, sv->precision, &probe) == QVD_OK) {
+        try {
+            qvd_options o = opt;
+            o.flags = (opt.flags & ~QVD_FLAG_AUTOTUNE) | QVD_FLAG_TIMING;
+            double best_t = 0;
+            for (uint32_t f : {vg.f_bits, vg.f_bits - 1, vg.f_bits + 1}) {
+                o.fixed_lanes = f;
+                double t = 1e300;
+                for (int rep = 0; rep < 2; rep++) {
+                    probe->zero_pending = sv->zero_pending;
+                    probe->zero_qubits = sv->zero_qubits;
+                    qvd_stats st;
+                    apply_plan(probe, ops, num_ops, o, st);
+                    t = std::min(t, st.stage_kernel_seconds + st.other_kernel_seconds);
+                }
+                if (f == vg.f_bits) best_t = t;
+                else if (t < 0.99 * best_t) {
+                    best = f;
+                    best_t = t;
+                }
+            }
+        } catch (...) {
+            (void)qvd_sv_destroy(probe);
+            throw;
+        }
+        (void)qvd_sv_destroy(probe);
+    }
+    std::lock_guard<std::mutex> lock(g_tune_sv.mu);
+    g_tune_sv.choice[key] = best;
+    return best;
+}
+
+}  // namespace
+}  // namespace qvd
+
 int qvd_sv_apply_ops(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qvd_options *options, qvd_stats *stats) {
     return guard([&] {
         require(sv != nullptr, "null state handle");
         const double t0 = now();
         qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0};
         if (options) opt = *options;
-        uint32_t variant = 0;
-        std::vector<Step> steps = make_plan(sv->n, sv->precision, ops, num_ops, opt, &variant);
-        const double t1 = now();
+        if (!opt.fixed_lanes && (opt.flags & QVD_FLAG_AUTOTUNE)) opt.fixed_lanes = tune_geometry(sv, ops, num_ops, opt);
         qvd_stats st{};
-        st.gates = num_ops;
-        st.plan_variant = variant;
-        // An empty plan leaves a pending |0...0> pending.
-        sv->bind();
-        if (!steps.empty()) {
-            const uint64_t zero = sv->zero_pending ? qubit_mask(sv->n) : sv->zero_qubits;
-            sv->zero_qubits = 0;  // until the plan has run
-            sv->zero_qubits = with_type(sv, [&](auto *s) { return run_plan(sv, s, steps, st, opt.flags, zero); });
-        }
-        const double t2 = now();
-        st.plan_seconds = t1 - t0;
-        st.gate_seconds = t2 - t0;
+        apply_plan(sv, ops, num_ops, opt, st);
+        st.gate_seconds = now() - t0;  // with the autotuner's probes when they ran
         if (stats) *stats = st;
     });
 }
@@ -2093,7 +2182,7 @@ int qvd_plan_features(uint32_t num_qubits, int precision, const qvd_op *ops, uin
         qvd_options opt{4, 0, QVD_PLANNER_HEURISTIC, 0, 0};
         if (options) opt = *options;
         std::vector<Step> steps = make_plan(num_qubits, precision, ops, num_ops, opt);
-        std::vector<float> f = plan_features(steps, num_qubits, precision);
+        std::vector<float> f = plan_features(steps, num_qubits, precision, opt.fixed_lanes);
         const uint64_t r = f.size() / QVD_PLAN_FEATURES;
         *rows = r;
         require(r <= capacity || features == nullptr, "feature buffer too small");
@@ -2124,7 +2213,8 @@ int qvd_plan_equivalent_variants(uint32_t num_qubits, int precision, const qvd_o
             *mask = 1u << opt.variant;  // LDS-tile plans: only the variant itself
             return;
         }
-        for (uint32_t var : equivalent_variants(plans, opt.variant, num_qubits, precision)) *mask |= 1u << var;
+        for (uint32_t var : equivalent_variants(plans, opt.variant, num_qubits, precision, opt.fixed_lanes))
+            *mask |= 1u << var;
     });
 }
 

@@ -18,7 +18,7 @@ template <typename T2> size_t vstage_lds_bytes();
 template <typename T2>
 hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const uint32_t *block_start,
                          const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
-                         hipStream_t st, bool from_zero = false);
+                         hipStream_t st, bool from_zero = false, uint32_t fixed_lanes = 0);
 
 // One item of a stage queue: tiles tile0, tile0 + step, ... (count of them)
 // of stage `desc`, which belongs to state `state` and may start once that

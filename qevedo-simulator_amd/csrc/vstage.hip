@@ -1026,19 +1026,25 @@ size_t vstage_lds_bytes() {
 template <typename T2>
 hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const uint32_t *block_start,
                          const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
-                         hipStream_t st, bool from_zero) {
+                         hipStream_t st, bool from_zero, uint32_t fixed_lanes) {
     const size_t lds = vstage_lds_bytes<T2>();
     using Kernel = void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *, const T2 *,
                             const StageOp *, const uint32_t *);
     // A stage that may start from |0...0> has a kernel of its own, so the
-    // others keep the code without that branch.
-    Kernel k, kz;
+    // others keep the code without that branch. The default geometry's
+    // fixed lane bits are compiled in; any other count is read from the
+    // descriptors.
+    Kernel k, kz, kf, kfz;
     if constexpr (sizeof(T2) == 8) {
         k = k_vstage<T2, 4, 4, 4, false>;
         kz = k_vstage<T2, 4, 4, 4, true>;
+        kf = k_vstage<T2, 4, 4, -1, false>;
+        kfz = k_vstage<T2, 4, 4, -1, true>;
     } else {
         k = k_vstage<T2, 3, 4, 3, false>;
         kz = k_vstage<T2, 3, 4, 3, true>;
+        kf = k_vstage<T2, 3, 4, -1, false>;
+        kfz = k_vstage<T2, 3, 4, -1, true>;
     }
     // The LDS size attribute is set once per device, before any launch a
     // caller may capture into a graph.
@@ -1052,7 +1058,15 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
         if (e != hipSuccess) return e;
         e = hipFuncSetAttribute((const void *)kz, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)kfz, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
         attr_set.fetch_or(bit);
+    }
+    if (fixed_lanes != 0 && fixed_lanes != vstage_geom<T2>().f_bits) {
+        k = kf;
+        kz = kfz;
     }
     if (from_zero) k = kz;
     k<<<grid, 64 << vstage_geom<T2>().w_bits, lds, st>>>(s, descs, ndesc, block_start, prog, coefs, ldsops, ints);
@@ -1118,10 +1132,10 @@ template size_t vstage_lds_bytes<float2>();
 template size_t vstage_lds_bytes<double2>();
 template hipError_t launch_vstage<float2>(float2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
                                           const float2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t,
-                                          bool);
+                                          bool, uint32_t);
 template hipError_t launch_vstage<double2>(double2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *,
                                            const double2 *, const StageOp *, const uint32_t *, unsigned, hipStream_t,
-                                           bool);
+                                           bool, uint32_t);
 template hipError_t launch_vstage_queue<float2>(float2 *, const VStageDesc *, const VQueueItem *, uint32_t, uint32_t *,
                                                 const VInstr *, const float2 *, const StageOp *, const uint32_t *,
                                                 unsigned, hipStream_t);

@@ -58,6 +58,7 @@ class Options(C.Structure):
         ("planner", C.c_uint32),
         ("flags", C.c_uint32),
         ("variant", C.c_uint32),
+        ("fixed_lanes", C.c_uint32),
     ]
 
 
@@ -85,6 +86,7 @@ class Stats(C.Structure):
         ("stage_bytes", C.c_uint64),
         ("other_bytes", C.c_uint64),
         ("skipped_tiles", C.c_uint64),
+        ("fixed_lanes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -358,7 +360,8 @@ class StateVector:
         _check(lib().qvd_sv_apply(self.h, _p(m.view(np.float64), C.c_double), _p(t, C.c_uint32), t.size, _p(cq, C.c_uint32), _p(cv, C.c_uint8), cq.size))
 
     def apply_ops(self, ops, max_fused_qubits: int = 4, tile_bits: int | None = 0, planner: int = PLANNER_HEURISTIC,
-                  timing: bool = False, variant: int = 0, dense_valu: bool = False, all_tiles: bool = False) -> dict:
+                  timing: bool = False, variant: int = 0, dense_valu: bool = False, all_tiles: bool = False,
+                  fixed_lanes: int = 0, autotune: bool = False) -> dict:
         """ops: [(targets, [(control, value)], matrix)]; tile_bits None = no blocking, 0 = default.
 
         planner PLANNER_LEARNED scores, with the installed cost model, the
@@ -367,11 +370,14 @@ class StateVector:
         (other op orders round differently); both are the heuristic when no
         model is installed. variant picks a heuristic variant directly.
         all_tiles: visit the register tiles known to be zero too
-        (QVD_FLAG_ALL_TILES)."""
+        (QVD_FLAG_ALL_TILES). fixed_lanes: the register-tile geometry (lane
+        bits pinned to the lowest qubits; 0 = the default); autotune: pick it
+        by timing, once per circuit class (QVD_FLAG_AUTOTUNE). The stats'
+        fixed_lanes is the geometry that ran."""
         arr = ops if isinstance(ops, OpArray) else OpArray(ops)
         flags = ((FLAG_TIMING if timing else 0) | (FLAG_DENSE_VALU if dense_valu else 0)
-                 | (FLAG_ALL_TILES if all_tiles else 0))
-        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, flags, variant)
+                 | (FLAG_ALL_TILES if all_tiles else 0) | (FLAG_AUTOTUNE if autotune else 0))
+        opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, flags, variant, fixed_lanes)
         st = Stats()
         _check(lib().qvd_sv_apply_ops(self.h, arr.arr, arr.n, C.byref(opt), C.byref(st)))
         return st.as_dict()
