@@ -58,16 +58,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(n: int):
+def measured_traffic(n: int, launches: int, fixed_lanes: int):
     """HBM bytes per stage launch of this workload (the plan's measured
     stage traffic over its launches), from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy pass of known
-    size), or None when no such measurement exists for this size."""
+    size), or None when no such measurement exists for this size, launch
+    count and register-tile geometry."""
     path = os.path.join(ROOT, "profiles", "k_vstage_random30_c64_traffic.json")
     if n != 30 or not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["hbm_bytes_per_launch"]
+        rec = json.load(f)
+    if rec.get("launches") != launches or rec.get("fixed_lanes") != fixed_lanes:
+        return None
+    return rec["hbm_bytes_per_launch"]
 
 
 def host_info() -> dict:
@@ -322,6 +326,11 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched hardware-efficient-ansatz workload")
     ap.add_argument("--batch-cpu-sample", type=int, default=64)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned planner")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default register-tile geometry instead of timing the candidates once per circuit")
+    ap.add_argument("--fixed-lanes", type=int, default=0,
+                    help="register-tile geometry to use instead of autotuning it (lane bits pinned to the lowest "
+                         "qubits; 0 = autotune unless --no-autotune)")
     ap.add_argument("--precision-form", type=int, default=32, metavar="QUBITS",
                     help="size of the precision form (fusion to 6-qubit GEMMs on matrix cores); 0 = skip")
     ap.add_argument("--batch-circuits", type=int, default=512)
@@ -346,7 +355,11 @@ def main() -> int:
     gates = library.random_circuit(n, depth, args.seed)
     ops = _native.OpArray(lower_gates(gates))
     log(f"{n} qubits, depth {depth}: {len(gates)} gates")
-    kw = {"max_fused_qubits": args.fusion, "tile_bits": args.tile_bits}
+    # The register-tile geometry (lane bits pinned to the lowest qubits) is
+    # autotuned: the first run of each plan times the candidates once on a
+    # scratch state (in the warm-up), every later run uses the choice.
+    kw = {"max_fused_qubits": args.fusion, "tile_bits": args.tile_bits,
+          "autotune": not args.no_autotune and not args.fixed_lanes, "fixed_lanes": args.fixed_lanes}
 
     sv = StateVector(n, "single")
     plans = {}
@@ -356,6 +369,12 @@ def main() -> int:
         f"stage kernels {h['stage_kernel_seconds']:.4f} s, other kernels {h['other_kernel_seconds']:.4f} s)")
     heuristic_amps = StateVector(n, "single")
     heuristic_amps.apply_ops(ops, **kw)
+    hd = h
+    if kw["autotune"]:
+        hd = time_plan(sv, ops, args.steps, min(args.warmup, 1), **{**kw, "autotune": False, "fixed_lanes": 0})
+        plans["heuristic_default_geometry"] = hd
+        log(f"heuristic on the default geometry ({hd['stats']['fixed_lanes']} fixed lane bits): {hd['seconds']:.4f} "
+            f"s/circuit ({hd['stats']['stages']} stages); autotuned {h['stats']['fixed_lanes']} fixed lane bits")
 
     la = time_plan(sv, ops, args.steps, min(args.warmup, 1), variant=24, **kw)
     plans["lookahead"] = la
@@ -392,6 +411,7 @@ def main() -> int:
                     "stage_launch_ms": head["stage_launch_seconds"] * 1e3, "plan_seconds": head["plan_seconds"],
                     "stage_bytes_per_circuit": head["stage_bytes"], "skipped_tiles": st["skipped_tiles"],
                     "all_tiles_seconds": at["seconds"],
+                    "fixed_lanes": st["fixed_lanes"], "default_geometry_seconds": hd["seconds"],
                     "all_tiles_roofline_achieved": at["stage_bytes"] / at["stage_kernel_seconds"] / 1e9}
 
     for _ in range(args.calibration_passes):
@@ -507,13 +527,17 @@ def main() -> int:
                    "fusion": "none inside register-tile stages (each stage program merges runs of 1-qubit gates "
                              f"per qubit); ops outside stages fused to at most {args.fusion} qubits",
                    "tile_bits": args.tile_bits or "register tile, 2^14 amplitudes",
+                   "geometry": f"{st['fixed_lanes']} lane bits pinned to the lowest qubits"
+                               + (" (autotuned once per circuit class)" if kw["autotune"] else
+                                  " (given)" if args.fixed_lanes else " (the default)"),
                    "planner": "learned (bit-identical to the heuristic)" if have_model else "heuristic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, st["stages"], st["fixed_lanes"]),
                      "algorithmic_bytes_per_launch": head["stage_bytes"] / max(st["stages"], 1)},
         "cpu_baseline": cpu,
         "error": error,
         "plans": {k: {"seconds": v["seconds"], "variant": v["stats"]["plan_variant"], "stages": v["stats"]["stages"],
+                      "fixed_lanes": v["stats"]["fixed_lanes"],
                       "stage_launch_ms": v["stage_launch_seconds"] * 1e3, "skipped_tiles": v["stats"]["skipped_tiles"],
                       **({"max_abs_diff_vs_heuristic": diffs[k]["max_abs"]} if k in diffs else {})}
                   for k, v in plans.items()},
