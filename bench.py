@@ -445,7 +445,7 @@ def main() -> int:
     if not args.no_check:
         ref = StateVector(n, "double")
         t = time.perf_counter()
-        ref.apply_ops(ops, **kw)
+        ref.apply_ops(ops, **{**kw, "fixed_lanes": 0})  # complex128 has its own geometries
         log(f"complex128 run: {time.perf_counter() - t:.3f} s")
         vs_c128 = sv.compare(ref)
         log(f"complex64 vs GPU complex128: max_abs {vs_c128['max_abs']:.3e} rel_l2 {vs_c128['rel_l2']:.3e}")
