@@ -1726,13 +1726,13 @@ uint32_t tune_geometry(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qv
     uint32_t best = 0;
     qvd_sv *probe = nullptr;
     if (qvd_sv_create(sv->n, sv->precision, &probe) == QVD_OK) {
-        try {
-            qvd_options o = opt;
-            o.flags = (opt.flags & ~QVD_FLAG_AUTOTUNE) | QVD_FLAG_TIMING;
-            double best_t = 0;
-            for (uint32_t f : {vg.f_bits, vg.f_bits - 1, vg.f_bits + 1}) {
-                o.fixed_lanes = f;
-                double t = 1e300;
+        qvd_options o = opt;
+        o.flags = (opt.flags & ~QVD_FLAG_AUTOTUNE) | QVD_FLAG_TIMING;
+        double best_t = 0;
+        for (uint32_t f : {vg.f_bits, vg.f_bits - 1, vg.f_bits + 1}) {
+            o.fixed_lanes = f;
+            double t = 1e300;
+            try {
                 for (int rep = 0; rep < 2; rep++) {
                     probe->zero_pending = sv->zero_pending;
                     probe->zero_qubits = sv->zero_qubits;
@@ -1740,15 +1740,18 @@ uint32_t tune_geometry(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qv
                     apply_plan(probe, ops, num_ops, o, st);
                     t = std::min(t, st.stage_kernel_seconds + st.other_kernel_seconds);
                 }
-                if (f == vg.f_bits) best_t = t;
-                else if (t < 0.99 * best_t) {
-                    best = f;
-                    best_t = t;
-                }
+            } catch (...) {
+                // a candidate that cannot run is skipped; the default's
+                // failure is the call's own
+                if (f != vg.f_bits) continue;
+                (void)qvd_sv_destroy(probe);
+                throw;
             }
-        } catch (...) {
-            (void)qvd_sv_destroy(probe);
-            throw;
+            if (f == vg.f_bits) best_t = t;
+            else if (t < 0.99 * best_t) {
+                best = f;
+                best_t = t;
+            }
         }
         (void)qvd_sv_destroy(probe);
     }
