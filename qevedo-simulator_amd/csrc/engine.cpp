@@ -1665,6 +1665,8 @@ int qvd_plan_ops(uint32_t num_qubits, int precision, const qvd_op *ops, uint64_t
         st.gates = num_ops;
         st.plan_seconds = now() - t0;
         st.plan_variant = variant;
+        st.fixed_lanes = opt.fixed_lanes ? opt.fixed_lanes
+                                         : (precision == QVD_C64 ? vstage_geom<float2>() : vstage_geom<double2>()).f_bits;
         count_plan(steps, st);
         if (stats) *stats = st;
     });

@@ -237,10 +237,10 @@ class OpArray:
 
 
 def plan_features(num_qubits: int, ops, precision: str = "single", variant: int = 0,
-                  max_fused_qubits: int = 4) -> np.ndarray:
+                  max_fused_qubits: int = 4, fixed_lanes: int = 0) -> np.ndarray:
     """Per register-tile stage features (rows x PLAN_FEATURES) of a plan variant."""
     arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-    opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0, variant)
+    opt = Options(max_fused_qubits, 0, PLANNER_HEURISTIC, 0, variant, fixed_lanes)
     prec = {"single": C64, "double": C128}[precision]
     rows = C.c_uint64()
     _check(lib().qvd_plan_features(num_qubits, prec, arr.arr, arr.n, C.byref(opt), None, 0, C.byref(rows)))
@@ -263,10 +263,10 @@ def equivalent_variants(num_qubits: int, ops, precision: str = "single", variant
 
 
 def plan_ops(num_qubits: int, ops, precision: str = "single", max_fused_qubits: int = 4, tile_bits: int | None = 0,
-             planner: int = PLANNER_HEURISTIC, variant: int = 0) -> dict:
-    """Plan statistics (stages, passes, fused ops) of apply_ops, on the host only."""
+             planner: int = PLANNER_HEURISTIC, variant: int = 0, fixed_lanes: int = 0) -> dict:
+    """Plan statistics (stages, passes, fused ops, geometry) of apply_ops, on the host only."""
     arr = ops if isinstance(ops, OpArray) else OpArray(ops)
-    opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0, variant)
+    opt = Options(max_fused_qubits, -1 if tile_bits is None else tile_bits, planner, 0, variant, fixed_lanes)
     st = Stats()
     _check(lib().qvd_plan_ops(num_qubits, {"single": C64, "double": C128}[precision], arr.arr, arr.n, C.byref(opt), C.byref(st)))
     return st.as_dict()

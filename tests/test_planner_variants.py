@@ -109,3 +109,21 @@ def test_op_wider_than_a_stage_gets_its_own_pass(variant):
     ops = [([0, 3, 6, 9, 12], [], m), ([1], [], np.eye(2)), ([0, 4, 8, 12, 15], [], m)]
     st = _native.plan_ops(16, ops, variant=variant)
     assert st["gates"] == 3 and st["passes"] >= 2 and st["passes"] - st["stages"] == 2
+
+
+def test_geometry_of_host_plans():
+    """Register-tile geometry on the host: the default and one fixed lane bit
+    fewer or more plan (and are reported); others are refused; features
+    follow the geometry (one row per register-tile stage)."""
+    ops = _native.OpArray(lower_gates(library.random_circuit(30, 20, 42)))
+    stages = {}
+    for f, prec in ((3, "single"), (4, "single"), (5, "single"), (2, "double"), (3, "double"), (4, "double")):
+        st = _native.plan_ops(30, ops, prec, fixed_lanes=f)
+        assert st["fixed_lanes"] == f and st["stages"] > 0
+        assert _native.plan_features(30, ops, prec, fixed_lanes=f).shape[0] == st["stages"]
+        stages[(f, prec)] = st["stages"]
+    assert _native.plan_ops(30, ops, "single")["fixed_lanes"] == 4
+    assert _native.plan_ops(30, ops, "double")["fixed_lanes"] == 3
+    for f, prec in ((2, "single"), (6, "single"), (1, "double"), (5, "double")):
+        with pytest.raises(_native.NativeError):
+            _native.plan_ops(30, ops, prec, fixed_lanes=f)
