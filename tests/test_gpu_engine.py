@@ -245,7 +245,7 @@ def _oracle_samples(amps, precision, uniforms):
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
-@pytest.mark.parametrize("n", [3, 15, 17, 19])
+@pytest.mark.parametrize("n", [1, 3, 5, 15, 17, 19])
 def test_sampling_bit_exact(precision, n):
     """measure.rs:104-174 on the same amplitudes and uniforms: identical
     outcome vectors (multi-chunk states, zero runs, boundary uniforms)."""
@@ -307,6 +307,20 @@ def test_sampling_rounding_leftovers(precision):
     want = _oracle_samples(stored, precision, u)
     assert np.array_equal(got, want)
     assert np.sum(want == 2 * chunk - 1) == 2
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_sampling_one_shot(precision):
+    """One uniform: one chunk walked by one thread."""
+    n = 16
+    r = rng(66)
+    a = random_state(r, n)
+    with StateVector(n, precision) as s:
+        s.set_amplitudes(a)
+        stored = s.amplitudes()
+        for u in (0.0, 0.37, 1 - 2 ** -53):
+            got = s.sample(np.array([u]))
+            assert np.array_equal(got, _oracle_samples(stored, precision, np.array([u]))), u
 
 
 def test_sampling_bit_exact_many_chunks():
