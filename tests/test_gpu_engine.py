@@ -323,6 +323,20 @@ def test_sampling_one_shot(precision):
             assert np.array_equal(got, _oracle_samples(stored, precision, np.array([u]))), u
 
 
+def test_sampling_more_shots_than_one_grid():
+    """5 * 10^6 shots: more than one thread per shot of the placement
+    kernel's grid (16384 workgroups of 256), so its threads stride."""
+    _note("sampling 5e6 shots")
+    n = 14
+    r = rng(71)
+    with StateVector(n, "single") as s:
+        s.set_amplitudes(random_state(r, n))
+        stored = s.amplitudes()
+        u = r.random(5_000_000)
+        got = s.sample(u)
+    assert np.array_equal(got, _oracle_samples(stored, "single", u))
+
+
 def test_sampling_bit_exact_many_chunks():
     """27 qubits in complex64: 4096 chunks, so the prefix over chunk sums
     runs over more than one of the device's staging tiles."""
