@@ -130,9 +130,11 @@ typedef struct qvd_stats {
     double other_kernel_seconds;  /* sum of full-state op kernel durations (QVD_FLAG_TIMING) */
     uint64_t plan_variant;        /* the plan variant that ran */
     uint64_t stage_bytes;         /* algorithmic HBM bytes of the register-tile stages: each reads and writes
-                                     the tiles it visits (a stage from |0...0> only writes) */
+                                     the tiles it visits (a stage from |0...0> only writes; amplitudes of
+                                     qubits whose zeros were never written are not read) */
     uint64_t other_bytes;         /* algorithmic HBM bytes of the other passes (full-state ops, writing a
-                                     pending |0...0>, a batch's initialisation and expectation values) */
+                                     pending |0...0> or unwritten zeros, a batch's initialisation and
+                                     expectation values) */
     uint64_t skipped_tiles;       /* register tiles not visited because they are known to be zero */
     uint64_t fixed_lanes;         /* register-tile geometry the stages ran with (qvd_options.fixed_lanes) */
 } qvd_stats;
@@ -153,7 +155,9 @@ int qvd_state_vector_fits(uint32_t num_qubits, int precision, int *fits, uint64_
 /* StateVector::zero (src/state.rs:39-49): |0...0> in HBM. The zeros are
  * written lazily: the next qvd_sv_apply_ops whose plan starts with a
  * register-tile stage makes that stage's tiles from |0...0> in registers (no
- * init pass, no loads), and any other call writes them first. */
+ * init pass, no loads). The tiles its stages skip as known to be zero may stay
+ * unwritten through any number of such applies; any other call writes them
+ * first. */
 int qvd_sv_create(uint32_t num_qubits, int precision, qvd_sv **out);
 int qvd_sv_destroy(qvd_sv *sv);
 int qvd_sv_num_qubits(const qvd_sv *sv, uint32_t *out);

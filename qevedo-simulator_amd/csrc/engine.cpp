@@ -100,6 +100,8 @@ struct qvd_sv {
     std::vector<double> stage_seconds;  // register-tile stage kernels of the last timed apply_ops
     bool zero_pending = false;          // the state is |0...0> but the amplitudes are not written yet
     uint64_t zero_qubits = 0;           // qubits known to be |0> (every amplitude with the bit set is zero)
+    uint64_t stale_qubits = 0;          // of those, qubits whose amplitudes have not been written since
+                                        // |0...0> (memory stale); every other amplitude is written
 
     size_t elem() const { return precision == QVD_C64 ? sizeof(float2) : sizeof(double2); }
     void bind() const { QVD_HIP(hipSetDevice(device)); }
@@ -129,7 +131,8 @@ struct qvd_batch {
     std::vector<uint32_t> qubits;     // each circuit's qubit count
     std::vector<uint32_t> fixed;      // each circuit's register-tile geometry (fixed lane bits)
     std::vector<uint32_t> circuit_stages;  // each circuit's number of stages
-    bool init_states = false;         // a first stage skips zero tiles: the replay writes every state first
+    bool init_states = false;         // the replay writes every state first (a circuit without stages, or one
+                                      // whose stages leave stale qubits); otherwise the stages' zeros are lazy
     uint64_t stage_bytes = 0, other_bytes = 0, skipped_tiles = 0;  // per replay
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -157,17 +160,21 @@ auto with_type(const qvd_sv *sv, F &&f) {
 // |0...0> is written lazily: qvd_sv_create and qvd_sv_set_zero only mark the
 // state, the next qvd_sv_apply_ops whose plan starts with a register-tile
 // stage makes that stage's tiles from |0...0> in registers (no init pass and
-// no tile loads), and any other use of the state writes the zeros first.
+// no tile loads), and a plan of register-tile stages only may leave the
+// amplitudes of stale qubits unwritten (run_plan). Any other use of the
+// state writes the missing zeros first.
 void materialize(qvd_sv *sv) {
-    if (!sv->zero_pending) return;
+    if (!sv->zero_pending && !sv->stale_qubits) return;
     sv->bind();
     with_type(sv, [&](auto *s) {
-        launch_init_zero(s, sv->len, sv->stream);
+        if (sv->zero_pending) launch_init_zero(s, sv->len, sv->stream);
+        else launch_zero_stale(s, sv->len, sv->stale_qubits, sv->stream);
         return 0;
     });
     QVD_HIP(hipGetLastError());
     sv->sync();
     sv->zero_pending = false;
+    sv->stale_qubits = 0;
 }
 
 // Every entry point that uses a state's amplitudes starts here. It may change
@@ -187,6 +194,17 @@ uint64_t after_op(uint64_t zero, const Op &op) {
     if (!op.diagonal)
         for (uint32_t q : op.targets) zero &= ~(uint64_t(1) << q);
     return zero;
+}
+
+// Tiles' worth of amplitudes a register-tile stage loads: none when it starts
+// from |0...0>; each stale qubit among the bits its visited tiles span halves
+// them (the tiles of skipped qubits are not visited).
+uint64_t stage_read_tiles(const VStageDesc &d) {
+    if (d.from_zero) return 0;
+    uint64_t span = 0;
+    for (uint32_t j = 0; j < d.L; j++) span |= uint64_t(1) << d.local_pos[j];
+    for (uint32_t j = 0; j < d.nglob; j++) span |= uint64_t(1) << d.glob_pos[j];
+    return d.num_tiles >> __builtin_popcountll(d.zero_mem & span);
 }
 
 // Drops from a stage's tile index the non-tile qubits known to be |0>: a tile
@@ -501,9 +519,25 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
     if (fixed_lanes) geom.f_bits = fixed_lanes;
     const bool skip_zero = (flags & QVD_FLAG_ALL_TILES) == 0;
     const uint64_t state_bytes = sv->len * sizeof(T2);
-    // A pending |0...0> is made in registers by the first step when that is a
-    // register-tile stage visiting every tile; otherwise it is written first.
-    bool write_zeros = sv->zero_pending;
+    // Amplitudes not written yet: all of them (a pending |0...0>), or those
+    // with a bit of a stale qubit set. A plan of register-tile stages only
+    // carries them: its first stage makes a pending |0...0> in registers,
+    // each stage sets the amplitudes of the stale qubits to zero instead of
+    // loading them and writes every tile it visits, and the qubits whose
+    // tiles it skips (known to be |0>) are the stale ones after it. Another
+    // plan makes a pending |0...0> in its first stage when that is a
+    // register-tile stage visiting every tile; otherwise, and for stale
+    // qubits, the zeros are written first.
+    bool all_reg = true;
+    for (const Step &step : steps) all_reg = all_reg && step.is_stage && step.stage.reg_tile;
+    bool pending = sv->zero_pending;
+    uint64_t stale = pending ? 0 : sv->stale_qubits;
+    bool write_zeros = false;
+    if ((pending && !(steps[0].is_stage && steps[0].stage.reg_tile)) || (!all_reg && stale)) {
+        write_zeros = true;
+        pending = false;
+        stale = 0;
+    }
     for (const Step &step : steps) {
         if (!step.is_stage) {
             zero = after_op(zero, step.op);
@@ -514,13 +548,20 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
             VProgram p = build_vprogram(step.stage, sv->n, geom, pend, vco, vlds, vints, true);
             require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
             p.desc.instr_off = (uint32_t)vins.size();
+            uint64_t glob = 0;  // the stage's non-tile qubits
+            for (uint32_t j = 0; j < p.desc.nglob; j++) glob |= uint64_t(1) << p.desc.glob_pos[j];
             const uint64_t dropped = skip_zero ? drop_zero_tiles(p.desc, zero) : 0;
             st.skipped_tiles += dropped;
-            if (sv->zero_pending && &step == &steps[0] && dropped == 0) {
-                p.desc.from_zero = 1;
-                write_zeros = false;
+            if (pending) {
+                // the first step
+                pending = false;
+                if (all_reg || dropped == 0) p.desc.from_zero = 1;
+                else write_zeros = true;
+            } else {
+                p.desc.zero_mem = stale;
             }
-            st.stage_bytes += (p.desc.from_zero ? 1 : 2) * (sizeof(T2) << p.desc.L) * p.desc.num_tiles;
+            if (all_reg) stale = skip_zero ? glob & zero : 0;
+            st.stage_bytes += (sizeof(T2) << p.desc.L) * (p.desc.num_tiles + stage_read_tiles(p.desc));
             vins.insert(vins.end(), p.instr.begin(), p.instr.end());
             vdescs.push_back(p.desc);
         } else {
@@ -532,8 +573,9 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
         for (const Op &op : step.stage.ops) zero = after_op(zero, op);
     }
     if (write_zeros) {
+        const uint64_t unwritten = sv->zero_pending ? sv->len : sv->len - (sv->len >> __builtin_popcountll(sv->stale_qubits));
         materialize(sv);
-        st.other_bytes += state_bytes;
+        st.other_bytes += unwritten * sizeof(T2);
     }
     std::vector<T2> vco_t(vco.size());
     for (size_t i = 0; i < vco.size(); i++) vco_t[i] = to_t2<T2>(vco[i]);
@@ -549,6 +591,7 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
         if (!step.is_stage) o_full.push_back(up.put(full_coefficients<T2>(step.op)));
     char *base = up.to_device(sv);
     sv->zero_pending = false;  // written above, or made by the first stage below
+    sv->stale_qubits = stale;
     const StageOp *dops = reinterpret_cast<const StageOp *>(base + o_ops);
     const T2 *dcoef = reinterpret_cast<const T2 *>(base + o_coef);
     const VInstr *dvins = reinterpret_cast<const VInstr *>(base + o_vins);
@@ -568,7 +611,7 @@ uint64_t run_plan(qvd_sv *sv, T2 *s, const std::vector<Step> &steps, qvd_stats &
             timer.begin(timer.stage);
             reg_launch.push_back(1);
             QVD_HIP(launch_vstage<T2>(s, dvdescs + vi, 1, nullptr, dvins, dvco, dvlds, dvints, grid, sv->stream,
-                                      d.from_zero != 0, geom.f_bits));
+                                      d.from_zero != 0 || d.zero_mem != 0, geom.f_bits));
             vi++;
             timer.end(timer.stage);
             st.stages++;
@@ -1086,9 +1129,14 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
         PendingDiag pend(npad);
         state_off[c] = off;
         // Every circuit starts from |0...0>: its stages skip the tiles known
-        // to be zero (a circuit whose first stage does must have its state
-        // written with zeros first).
+        // to be zero. Its first stage makes its tiles in registers; the
+        // tiles it skips are left unwritten (stale), and each later stage
+        // sets the amplitudes of the stale qubits to zero instead of loading
+        // them and leaves the qubits it skips stale. A circuit with stale
+        // qubits after its last stage (or no stages) needs its state written
+        // first; the replay then writes every state and no stage is lazy.
         uint64_t zero = (opt.flags & QVD_FLAG_ALL_TILES) ? 0 : qubit_mask(npad);
+        uint64_t stale = 0;
         for (const Step &st : steps) {
             if (!st.is_stage || !st.stage.reg_tile)
                 fail(QVD_EUNSUPPORTED, "circuit " + std::to_string(c) +
@@ -1097,14 +1145,17 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
             require(p.desc.ccount * sizeof(T2) <= VCONST_BYTES, "internal: stage coefficients exceed the LDS budget");
             p.desc.instr_off = (uint32_t)vins.size();
             p.desc.state_off = off;
-            const uint64_t dropped = drop_zero_tiles(p.desc, zero);
-            b->skipped_tiles += dropped;
-            if (stage_descs[c].empty() && dropped) b->init_states = true;
+            uint64_t glob = 0;  // the stage's non-tile qubits
+            for (uint32_t j = 0; j < p.desc.nglob; j++) glob |= uint64_t(1) << p.desc.glob_pos[j];
+            b->skipped_tiles += drop_zero_tiles(p.desc, zero);
+            p.desc.zero_mem = stale;
+            stale = glob & zero;
             for (const Op &op : st.stage.ops) zero = after_op(zero, op);
             vins.insert(vins.end(), p.instr.begin(), p.instr.end());
             stage_descs[c].push_back(p.desc);
             b->kernels += st.stage.ops.size();
         }
+        if (steps.empty() || stale) b->init_states = true;
         b->gates += ci.num_ops;
         ExpDesc &e = ed[c];
         e = expect_desc(npad, ci.num_qubits);
@@ -1138,12 +1189,9 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     std::vector<VStageDesc> all_descs;
     std::vector<VQueueItem> items;
     std::vector<uint64_t> queued(count, 0);
-    bool any_empty = false;
     uint64_t remaining = 0;  // tiles of the queue from the current stage on
-    for (uint32_t c = 0; c < count; c++) {
-        any_empty |= stage_descs[c].empty();
+    for (uint32_t c = 0; c < count; c++)
         for (const VStageDesc &d : stage_descs[c]) remaining += d.num_tiles;
-    }
     unsigned resident = 0;
     QVD_HIP(vstage_resident_groups<T2>(&resident));
     for (uint32_t r = 0;; r++) {
@@ -1152,10 +1200,10 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
             if (r >= stage_descs[c].size()) continue;
             any = true;
             VStageDesc d = stage_descs[c][r];
-            // The first stage makes its tiles from |0...0> unless the states
-            // are written first (a first stage that skips zero tiles).
+            // Lazy zeros unless the states are written first.
             d.from_zero = r == 0 && !b->init_states;
-            b->stage_bytes += (d.from_zero ? 1 : 2) * (sizeof(T2) << d.L) * d.num_tiles;
+            if (b->init_states) d.zero_mem = 0;
+            b->stage_bytes += (sizeof(T2) << d.L) * (d.num_tiles + stage_read_tiles(d));
             const uint32_t di = (uint32_t)all_descs.size();
             all_descs.push_back(d);
             // Large items while much work remains (each item costs a fetch, a
@@ -1177,7 +1225,7 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     require(items.size() < (uint64_t(1) << 32), "batch too large for one stage queue");
     // Initialisation (when it runs) writes every state; the expectation
     // values read every state.
-    b->other_bytes = ((any_empty || b->init_states) ? 2 : 1) * off * sizeof(T2);
+    b->other_bytes = (b->init_states ? 2 : 1) * off * sizeof(T2);
     b->nitems = (uint32_t)items.size();
     b->grid = (unsigned)std::min<uint64_t>(resident, std::max<uint64_t>(items.size(), 1));
     // Upload.
@@ -1214,7 +1262,7 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
         // Every circuit's first stage starts from |0...0> without loading,
         // unless some first stage skips zero tiles or a circuit has no stage:
         // then every state is written first.
-        if (any_empty || b->init_states) launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
+        if (b->init_states) launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
         launch_zero_words(b->sync.as<uint32_t>(), VQUEUE_COUNTERS + count, b->stream);
         QVD_HIP(hipGetLastError());
         if (b->nitems)
@@ -1568,6 +1616,7 @@ int qvd_sv_set_zero(qvd_sv *sv) {
         require(sv != nullptr, "null state handle");
         sv->zero_pending = true;
         sv->zero_qubits = qubit_mask(sv->n);
+        sv->stale_qubits = 0;
     });
 }
 
@@ -1580,6 +1629,7 @@ int qvd_sv_copy(qvd_sv *dst, const qvd_sv *src) {
         // dst is overwritten whole: a pending |0...0> in src is copied as pending.
         dst->zero_pending = src->zero_pending;
         dst->zero_qubits = src->zero_qubits;
+        dst->stale_qubits = src->stale_qubits;  // stale memory is copied as stale
         if (src->zero_pending) return;
         src->sync();
         dst->bind();
@@ -1596,6 +1646,7 @@ int qvd_sv_set_amplitudes(qvd_sv *sv, const double *amplitudes, uint64_t len) {
         require(len == sv->len, "expected " + std::to_string(sv->len) + " amplitudes, got " + std::to_string(len));
         sv->zero_pending = false;  // every amplitude is written below
         sv->zero_qubits = 0;
+        sv->stale_qubits = 0;
         sv->bind();
         const uint64_t chunk = std::min<uint64_t>(len, uint64_t(1) << 24);
         DBuf stage(chunk * sizeof(double2));
@@ -1738,6 +1789,7 @@ uint32_t tune_geometry(qvd_sv *sv, const qvd_op *ops, uint64_t num_ops, const qv
                 for (int rep = 0; rep < 2; rep++) {
                     probe->zero_pending = sv->zero_pending;
                     probe->zero_qubits = sv->zero_qubits;
+                    probe->stale_qubits = sv->stale_qubits;
                     qvd_stats st;
                     apply_plan(probe, ops, num_ops, o, st);
                     t = std::min(t, st.stage_kernel_seconds + st.other_kernel_seconds);

@@ -42,6 +42,8 @@ hipError_t noalloc_scratch(void **ptr);
 constexpr uint32_t PAULI_TERMS = 16;
 
 template <typename T2> void launch_init_zero(T2 *s, uint64_t len, hipStream_t st);
+// Zero over every amplitude whose index has a bit of `stale` set.
+template <typename T2> void launch_zero_stale(T2 *s, uint64_t len, uint64_t stale, hipStream_t st);
 template <typename T2> void launch_probabilities(const T2 *s, uint64_t len, double *out, hipStream_t st);
 template <typename T2>
 void launch_expect_pauli(const T2 *s, uint64_t len, uint64_t x, const uint64_t *zs, uint32_t nterms, double *partial,

@@ -71,6 +71,20 @@ __global__ void k_init_zero(T2 *__restrict__ s, uint64_t len) {
     }
 }
 
+// Writes zero over every amplitude whose index has a bit of `stale` set.
+template <typename T2>
+__global__ void k_zero_stale(T2 *__restrict__ s, uint64_t len, uint64_t stale) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        if (i & stale) {
+            T2 v;
+            v.x = 0;
+            v.y = 0;
+            s[i] = v;
+        }
+    }
+}
+
 template <typename T2>
 __global__ void k_widen(const T2 *__restrict__ s, double2 *__restrict__ out, uint64_t count) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -981,6 +995,10 @@ void launch_init_zero(T2 *s, uint64_t len, hipStream_t st) {
     k_init_zero<T2><<<grid_for(len, 256), 256, 0, st>>>(s, len);
 }
 template <typename T2>
+void launch_zero_stale(T2 *s, uint64_t len, uint64_t stale, hipStream_t st) {
+    k_zero_stale<T2><<<grid_for(len, 256), 256, 0, st>>>(s, len, stale);
+}
+template <typename T2>
 void launch_widen(const T2 *s, double2 *out, uint64_t count, hipStream_t st) {
     k_widen<T2><<<grid_for(count, 256), 256, 0, st>>>(s, out, count);
 }
@@ -1155,6 +1173,7 @@ hipError_t launch_sample_walk(const T2 *s, uint64_t len, uint64_t chunk, uint64_
 
 #define QVD_INSTANTIATE(T2)                                                                                     \
     template void launch_init_zero<T2>(T2 *, uint64_t, hipStream_t);                                            \
+    template void launch_zero_stale<T2>(T2 *, uint64_t, uint64_t, hipStream_t);                                 \
     template void launch_widen<T2>(const T2 *, double2 *, uint64_t, hipStream_t);                               \
     template void launch_narrow<T2>(const double2 *, T2 *, uint64_t, hipStream_t);                              \
     template void launch_scale<T2>(T2 *, uint64_t, double, hipStream_t);                                        \

@@ -157,6 +157,8 @@ struct VStageDesc {
     uint32_t cbegin, ccount;         // the stage's coefficients, copied to LDS once per workgroup
     uint32_t from_zero;              // nonzero: the state is |0...0> before this stage (its tiles are not loaded)
     uint32_t fixed_lanes;            // F of the geometry the program was built for
+    uint64_t zero_mem;               // stale qubits: an amplitude with one of these bits set is zero and its memory
+                                     // has not been written since |0...0> (it is not loaded)
 };
 
 // LDS budget of a stage's coefficient block, in bytes.

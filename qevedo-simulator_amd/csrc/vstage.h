@@ -13,12 +13,13 @@ template <typename T2> size_t vstage_lds_bytes();
 // Runs ndesc register-tile stages in one launch of `grid` workgroups:
 // descriptor i (device memory) owns workgroups block_start[i] ..
 // block_start[i + 1] - 1 (device memory, ndesc + 1 entries; unused and may
-// be null when ndesc == 1). All pointers are device pointers. from_zero: a
-// descriptor may have from_zero set (its tiles are made, not loaded).
+// be null when ndesc == 1). All pointers are device pointers. lazy_zero: a
+// descriptor may have from_zero set (its tiles are made, not loaded) or
+// stale qubits in zero_mem (their amplitudes are set to zero, not loaded).
 template <typename T2>
 hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const uint32_t *block_start,
                          const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
-                         hipStream_t st, bool from_zero = false, uint32_t fixed_lanes = 0);
+                         hipStream_t st, bool lazy_zero = false, uint32_t fixed_lanes = 0);
 
 // One item of a stage queue: tiles tile0, tile0 + step, ... (count of them)
 // of stage `desc`, which belongs to state `state` and may start once that

@@ -614,7 +614,8 @@ __device__ __forceinline__ void inline_coefs(const QVD_CONST VInstr &v, T2 (&c)[
 // Runs stage d on its tiles t_begin, t_begin + step, ... below t_end; s is
 // the first amplitude of the stage's state. With MAY_ZERO the stage may be
 // marked from_zero: its tiles are then made in registers (|0...0>), not
-// loaded. FBC is the number of fixed lane bits, or -1 to read it from the
+// loaded; or it may name stale qubits (zero_mem), whose amplitudes it sets to
+// zero instead of loading them. FBC is the number of fixed lane bits, or -1 to read it from the
 // descriptor (it enters only the offsets of the role maps, not the gate
 // handlers).
 template <typename T2, int WB, int RB, int FBC, bool MAY_ZERO>
@@ -694,15 +695,31 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
         for (uint32_t j = 0; j < d.nglob; j++) base |= ((t >> j) & 1) << d.glob_pos[j];
         return base;
     };
+    // With MAY_ZERO, amplitudes whose index has a bit of d.zero_mem set are
+    // zero and their memory is stale: they are set, not loaded.
     auto load = [&](T2(&dst)[R], uint64_t base) {
         const T2 *src = s + base + t_off0;
+        const uint64_t zm = MAY_ZERO ? d.zero_mem : 0;
+        if (zm == 0) {
 #pragma unroll
-        for (int j = 0; j < R; j++) {
-            uint64_t o = 0;
+            for (int j = 0; j < R; j++) {
+                uint64_t o = 0;
 #pragma unroll
-            for (int b = 0; b < RB; b++)
-                if ((j >> b) & 1) o += p0[b];
-            dst[j] = src[o];
+                for (int b = 0; b < RB; b++)
+                    if ((j >> b) & 1) o += p0[b];
+                dst[j] = src[o];
+            }
+        } else {
+            const bool tz = ((base | t_off0) & zm) != 0;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                uint64_t o = 0;
+#pragma unroll
+                for (int b = 0; b < RB; b++)
+                    if ((j >> b) & 1) o += p0[b];
+                if (tz || (o & zm) != 0) dst[j] = mk<T2>(0, 0);
+                else dst[j] = src[o];
+            }
         }
     };
 
@@ -1026,12 +1043,12 @@ size_t vstage_lds_bytes() {
 template <typename T2>
 hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const uint32_t *block_start,
                          const VInstr *prog, const T2 *coefs, const StageOp *ldsops, const uint32_t *ints, unsigned grid,
-                         hipStream_t st, bool from_zero, uint32_t fixed_lanes) {
+                         hipStream_t st, bool lazy_zero, uint32_t fixed_lanes) {
     const size_t lds = vstage_lds_bytes<T2>();
     using Kernel = void (*)(T2 *, const VStageDesc *, uint32_t, const uint32_t *, const VInstr *, const T2 *,
                             const StageOp *, const uint32_t *);
-    // A stage that may start from |0...0> has a kernel of its own, so the
-    // others keep the code without that branch. The default geometry's
+    // A stage that starts from |0...0> or names stale qubits has a kernel of
+    // its own, so the others keep the code without those branches. The default geometry's
     // fixed lane bits are compiled in; any other count is read from the
     // descriptors.
     Kernel k, kz, kf, kfz;
@@ -1068,7 +1085,7 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
         k = kf;
         kz = kfz;
     }
-    if (from_zero) k = kz;
+    if (lazy_zero) k = kz;
     k<<<grid, 64 << vstage_geom<T2>().w_bits, lds, st>>>(s, descs, ndesc, block_start, prog, coefs, ldsops, ints);
     return hipGetLastError();
 }

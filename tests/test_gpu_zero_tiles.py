@@ -1,9 +1,13 @@
 """Register tiles known to be zero are skipped: a qubit that no non-diagonal
 op has targeted since |0...0> is still |0>, so a stage whose tile leaves it
-out neither reads nor writes the tiles with its bit set. Every case here
-compares the result with the same plan visiting every tile
-(QVD_FLAG_ALL_TILES; np.array_equal, under which -0.0 equals 0.0) and with
-the oracle's FP64 state."""
+out neither reads nor writes the tiles with its bit set. Those zeros are not
+written either: a plan of register-tile stages makes |0...0> in its first
+stage's registers and leaves the skipped qubits' amplitudes stale in memory;
+later stages set them to zero instead of loading them, and any other use of
+the state writes them first. Every case here compares the result with the
+same plan visiting every tile (QVD_FLAG_ALL_TILES; np.array_equal, under
+which -0.0 equals 0.0) and with the oracle's FP64 state; the cases of stale
+memory fill it with nonzero values first."""
 import numpy as np
 import pytest
 
@@ -64,8 +68,9 @@ def test_second_apply_continues_from_the_tracked_qubits():
 
 def test_batch_skips_zero_tiles():
     """Batches start every circuit from |0...0>: their first stages compute
-    only the tiles that can be nonzero (the states are written with zeros
-    first), with the same results as visiting every tile."""
+    only the tiles that can be nonzero (in registers; the states are not
+    written first), with the same results as visiting every tile. The second
+    run finds the first run's amplitudes in memory."""
     rng = np.random.Generator(np.random.PCG64(31))
     sizes = [int(x) for x in rng.integers(14, 23, size=24)]
     circuits = []
@@ -76,10 +81,113 @@ def test_batch_skips_zero_tiles():
             Batch([(n, ops) for n, ops, _ in circuits], "single", all_tiles=True) as a:
         out, ref = b.run(), a.run()
         assert b.stats["skipped_tiles"] > 0 and a.stats["skipped_tiles"] == 0
+        again = b.run()
         for i, (n, _, g) in enumerate(circuits):
+            assert np.array_equal(again[i][0], out[i][0]) and np.array_equal(again[i][1], out[i][1])
             print(f"circuit {i}: {n} qubits", flush=True)
             assert np.array_equal(b.amplitudes(i), a.amplitudes(i))
             assert np.array_equal(out[i][0], ref[i][0]) and np.array_equal(out[i][1], ref[i][1])
             want = oracle_statevector(n, g)
             got = b.amplitudes(i).astype(np.complex128)
             assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
+
+
+def _dirty_zero(sv, n, seed):
+    """Nonzero values in every amplitude's memory, then |0...0> (pending, not
+    written): memory that no stage may read as amplitudes."""
+    r = np.random.default_rng(seed)
+    a = r.standard_normal(1 << n) + 1j * r.standard_normal(1 << n)
+    sv.set_amplitudes(a / np.linalg.norm(a))
+    sv.set_zero()
+
+
+def _partial(n):
+    """Qubits 0..n-5 only: every stage skips the tiles of the top four, whose
+    amplitudes stay stale after the apply."""
+    return library.random_circuit(n - 4, 6, 11)
+
+
+@pytest.mark.parametrize("precision,tol", [("single", 1e-5), ("double", 1e-12)])
+def test_stale_amplitudes_read_as_zero(precision, tol):
+    """An apply that leaves qubits stale writes no zeros; reading the state
+    writes them."""
+    n = 20
+    gates = _partial(n)
+    with StateVector(n, precision) as sv, StateVector(n, precision) as full:
+        _dirty_zero(sv, n, 1)
+        _dirty_zero(full, n, 2)
+        st = sv.apply_ops(_native.OpArray(lower_gates(gates)))
+        full.apply_ops(_native.OpArray(lower_gates(gates)), all_tiles=True)
+        assert st["other_bytes"] == 0 and st["skipped_tiles"] > 0
+        got = sv.amplitudes()
+        assert np.array_equal(got, full.amplitudes())
+        assert not np.any(got.reshape(16, -1)[1:])  # every amplitude with a top-four bit set
+    want = oracle_statevector(n, gates)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("single", 1e-5), ("double", 1e-12)])
+def test_stages_over_stale_amplitudes(precision, tol):
+    """A second apply whose stages bring the stale qubits in: their
+    amplitudes are set to zero, not loaded, and the apply writes no zeros."""
+    n = 20
+    first, second = _partial(n), _late(n)
+    with StateVector(n, precision) as sv, StateVector(n, precision) as full:
+        _dirty_zero(sv, n, 3)
+        _dirty_zero(full, n, 4)
+        a = sv.apply_ops(_native.OpArray(lower_gates(first)))
+        b = sv.apply_ops(_native.OpArray(lower_gates(second)))
+        full.apply_ops(_native.OpArray(lower_gates(first)), all_tiles=True)
+        c = full.apply_ops(_native.OpArray(lower_gates(second)), all_tiles=True)
+        assert a["other_bytes"] == 0 and b["other_bytes"] == 0
+        assert b["stage_bytes"] < c["stage_bytes"]
+        got = sv.amplitudes()
+        assert np.array_equal(got, full.amplitudes())
+    want = oracle_statevector(n, first + second)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
+
+
+def test_every_tile_over_stale_amplitudes():
+    """Visiting every tile after an apply that left qubits stale: the stale
+    qubits' tiles are visited too, and read as zero."""
+    n = 20
+    first, second = _partial(n), _late(n)
+    with StateVector(n) as sv, StateVector(n) as ref:
+        _dirty_zero(sv, n, 5)
+        sv.apply_ops(_native.OpArray(lower_gates(first)))
+        st = sv.apply_ops(_native.OpArray(lower_gates(second)), all_tiles=True)
+        assert st["other_bytes"] == 0 and st["skipped_tiles"] == 0
+        ref.apply_ops(_native.OpArray(lower_gates(first)), all_tiles=True)
+        ref.apply_ops(_native.OpArray(lower_gates(second)), all_tiles=True)
+        assert np.array_equal(sv.amplitudes(), ref.amplitudes())
+
+
+def test_other_plans_and_entry_points_write_stale_zeros():
+    """After an apply that left qubits stale: a plan with a full-state pass
+    (no blocking) writes the zeros first, and so does a single gate (an
+    entry point of its own); a copy carries the stale qubits to its target."""
+    n = 20
+    first = _partial(n)
+    h = np.array([[1, 1], [1, -1]], dtype=np.complex128) / np.sqrt(2)
+    with StateVector(n) as sv, StateVector(n) as cp, StateVector(n) as ref:
+        for k, case in enumerate(["unblocked", "gate", "copy"]):
+            _dirty_zero(sv, n, 10 + k)
+            _dirty_zero(cp, n, 20 + k)
+            ref.set_zero()
+            sv.apply_ops(_native.OpArray(lower_gates(first)))
+            ref.apply_ops(_native.OpArray(lower_gates(first)), all_tiles=True)
+            tgt = sv
+            if case == "unblocked":
+                st = sv.apply_ops(_native.OpArray(lower_gates(_late(n))), tile_bits=None)
+                ref.apply_ops(_native.OpArray(lower_gates(_late(n))), tile_bits=None)
+                assert st["other_bytes"] > 0
+            elif case == "gate":
+                sv.apply(h, [n - 1])
+                ref.apply(h, [n - 1])
+            else:
+                cp.copy_from(sv)
+                cp.apply_ops(_native.OpArray(lower_gates(_late(n))))
+                ref.apply_ops(_native.OpArray(lower_gates(_late(n))), all_tiles=True)
+                tgt = cp
+            print(f"case {case}", flush=True)
+            assert np.array_equal(tgt.amplitudes(), ref.amplitudes())
