@@ -191,3 +191,70 @@ def test_other_plans_and_entry_points_write_stale_zeros():
                 tgt = cp
             print(f"case {case}", flush=True)
             assert np.array_equal(tgt.amplitudes(), ref.amplitudes())
+
+
+def test_sampling_a_state_with_stale_qubits():
+    """Both sampling entry points on a state whose apply left qubits stale:
+    the outcomes of the same amplitudes made visiting every tile, and the
+    oracle's on those amplitudes (measure.rs:104-174)."""
+    import ctypes as C
+    from oracle import oracle
+
+    n = 20
+    gates = _partial(n)
+    u = np.random.default_rng(7).random(100_000)
+    u[:3] = [0.0, 0.5, 1 - 2 ** -53]
+    with StateVector(n) as ref:
+        ref.apply_ops(_native.OpArray(lower_gates(gates)), all_tiles=True)
+        stored = ref.amplitudes()
+        want = ref.sample(u)
+    with oracle.OracleState(n, "single") as o:
+        o.set_amplitudes(stored)
+        assert np.array_equal(o.sample(u), want)
+    for entry in ["sample", "noalloc"]:
+        print(f"sampling through {entry}", flush=True)
+        with StateVector(n) as sv:
+            _dirty_zero(sv, n, 30)
+            st = sv.apply_ops(_native.OpArray(lower_gates(gates)))
+            assert st["other_bytes"] == 0
+            if entry == "sample":
+                got = sv.sample(u)
+            else:
+                got = np.empty(u.size, dtype=np.uint64)
+                rc = _native.lib().qvd_sv_sample_noalloc(sv.h, _native._p(u, C.c_double), u.size,
+                                                         _native._p(got, C.c_uint64))
+                assert rc == 0, _native.lib().qvd_last_error()
+            assert np.array_equal(got, want)
+            assert np.array_equal(sv.amplitudes(), stored)
+
+
+@pytest.mark.parametrize("precision,tol", [("single", 1e-5), ("double", 1e-12)])
+@pytest.mark.parametrize("stale_end", [False, True])
+def test_batch_lazy_and_written_states(precision, tol, stale_end):
+    """A batch whose circuits all end visiting every tile writes no state
+    before its stages; one with a circuit that ends with stale qubits (it
+    touches only its lowest qubits) writes every state first. Either way the
+    results equal every tile visited, on two runs in a row, and the
+    oracle's."""
+    circuits = []
+    for i, n in enumerate([15, 17, 20, 22]):
+        g = library.random_circuit(n, 6, 60 + i) if i % 2 else _late(n)
+        circuits.append((n, g))
+    if stale_end:
+        circuits.append((20, _partial(20)))
+    with Batch([(n, lower_gates(g)) for n, g in circuits], precision) as b, \
+            Batch([(n, lower_gates(g)) for n, g in circuits], precision, all_tiles=True) as a:
+        out, ref = b.run(), a.run()
+        assert b.stats["skipped_tiles"] > 0
+        state_bytes = sum((8 if precision == "single" else 16) << max(n, 14 if precision == "single" else 13)
+                          for n, _ in circuits)
+        assert b.stats["other_bytes"] == (2 if stale_end else 1) * state_bytes
+        again = b.run()
+        for i, (n, g) in enumerate(circuits):
+            print(f"circuit {i}: {n} qubits", flush=True)
+            assert np.array_equal(b.amplitudes(i), a.amplitudes(i))
+            for r in (out, again):
+                assert np.array_equal(r[i][0], ref[i][0]) and np.array_equal(r[i][1], ref[i][1])
+            want = oracle_statevector(n, g)
+            got = b.amplitudes(i).astype(np.complex128)
+            assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
