@@ -136,10 +136,12 @@ struct qvd_batch {
     uint64_t stage_bytes = 0, other_bytes = 0, skipped_tiles = 0;  // per replay
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    double *host_out = nullptr;  // pinned: a run's <Z>, <ZZ> and the queue's error word, copied back together
     ~qvd_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
+        if (host_out) (void)hipHostFree(host_out);
     }
 };
 
@@ -1083,10 +1085,12 @@ void count_plan(const std::vector<Step> &steps, qvd_stats &st) {
 }
 
 // Fails when a wait of the batch's stage queue ran out (after a replay).
+const char *const QUEUE_TIMEOUT = "a batch stage waited more than 10 s for its previous stage; results are invalid";
+
 void check_queue(const qvd_batch *b) {
     uint32_t err = 0;
     QVD_HIP(hipMemcpy(&err, b->sync.as<uint32_t>() + VQUEUE_ERROR, sizeof(err), hipMemcpyDeviceToHost));
-    if (err) fail(QVD_EHIP, "a batch stage waited more than 10 s for its previous stage; results are invalid");
+    if (err) fail(QVD_EHIP, QUEUE_TIMEOUT);
 }
 
 // Builds a batch: plans each circuit on max(n, L) qubits (L = the register
@@ -1251,6 +1255,9 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     b->seg_sums.alloc(std::max<uint64_t>(seg_off, 1) * sizeof(double));
     b->z.alloc(std::max<uint64_t>(b->num_z, 1) * sizeof(double));
     b->zz.alloc(std::max<uint64_t>(b->num_zz, 1) * sizeof(double));
+    if (b->host_out) QVD_HIP(hipHostFree(b->host_out));
+    b->host_out = nullptr;
+    QVD_HIP(hipHostMalloc((void **)&b->host_out, (b->num_z + b->num_zz + 1) * sizeof(double), hipHostMallocDefault));
     const uint64_t bytes = b->len * sizeof(T2);
     const uint64_t cap = state_cap_bytes();
     if (cap && bytes > cap) fail(QVD_ENOMEM, "batch states of " + std::to_string(bytes) + " bytes exceed QVD_MAX_STATE_BYTES");
@@ -1260,8 +1267,8 @@ void build_batch(qvd_batch *b, const qvd_circuit *circuits, uint32_t count, cons
     T2 *s = b->amps.as<T2>();
     auto enqueue = [&]() {
         // Every circuit's first stage starts from |0...0> without loading,
-        // unless some first stage skips zero tiles or a circuit has no stage:
-        // then every state is written first.
+        // unless a circuit ends with stale qubits or has no stage: then every
+        // state is written first.
         if (b->init_states) launch_batch_init<T2>(s, b->len, b->basis.as<uint64_t>(), count, b->stream);
         launch_zero_words(b->sync.as<uint32_t>(), VQUEUE_COUNTERS + count, b->stream);
         QVD_HIP(hipGetLastError());
@@ -2208,13 +2215,23 @@ int qvd_batch_run(qvd_batch *batch, double *z, double *zz, qvd_stats *stats) {
         QVD_HIP(hipSetDevice(batch->device));
         const double t0 = now();
         QVD_HIP(hipGraphLaunch(batch->exec, batch->stream));
+        // The outputs and the queue's error word come back in one pinned
+        // buffer, queued behind the replay, so one wait covers everything.
+        double *hz = batch->host_out, *hzz = hz + batch->num_z;
+        uint32_t *herr = reinterpret_cast<uint32_t *>(hzz + batch->num_zz);
+        if (batch->num_z)
+            QVD_HIP(hipMemcpyAsync(hz, batch->z.p, batch->num_z * sizeof(double), hipMemcpyDeviceToHost,
+                                   batch->stream));
+        if (batch->num_zz)
+            QVD_HIP(hipMemcpyAsync(hzz, batch->zz.p, batch->num_zz * sizeof(double), hipMemcpyDeviceToHost,
+                                   batch->stream));
+        QVD_HIP(hipMemcpyAsync(herr, batch->sync.as<uint32_t>() + VQUEUE_ERROR, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, batch->stream));
         QVD_HIP(hipStreamSynchronize(batch->stream));
         const double t1 = now();
-        check_queue(batch);
-        if (z && batch->num_z)
-            QVD_HIP(hipMemcpy(z, batch->z.p, batch->num_z * sizeof(double), hipMemcpyDeviceToHost));
-        if (zz && batch->num_zz)
-            QVD_HIP(hipMemcpy(zz, batch->zz.p, batch->num_zz * sizeof(double), hipMemcpyDeviceToHost));
+        if (*herr) fail(QVD_EHIP, QUEUE_TIMEOUT);
+        if (z && batch->num_z) std::memcpy(z, hz, batch->num_z * sizeof(double));
+        if (zz && batch->num_zz) std::memcpy(zz, hzz, batch->num_zz * sizeof(double));
         if (stats) {
             qvd_stats st{};
             st.gates = batch->gates;

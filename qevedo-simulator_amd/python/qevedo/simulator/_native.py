@@ -499,6 +499,9 @@ class Batch:
         nz, nzz = C.c_uint64(), C.c_uint64()
         _check(lib().qvd_batch_outputs(self.h, C.byref(nz), C.byref(nzz)))
         self.num_z, self.num_zz = nz.value, nzz.value
+        # Each circuit's slice of the run's <Z> and <ZZ> arrays.
+        self._z_off = np.cumsum([0] + self.sizes).tolist()
+        self._zz_off = np.cumsum([0] + [n - 1 for n in self.sizes]).tolist()
         self.stats = {}
         self.precision = precision
 
@@ -521,17 +524,15 @@ class Batch:
         return out
 
     def run(self) -> list:
+        """Per circuit, (<Z_q> for every qubit, <Z_q Z_{q+1}>): views into
+        arrays that this run allocates, so later runs leave them as they are."""
         z = np.zeros(max(self.num_z, 1), dtype=np.float64)
         zz = np.zeros(max(self.num_zz, 1), dtype=np.float64)
         st = Stats()
         _check(lib().qvd_batch_run(self.h, _p(z, C.c_double), _p(zz, C.c_double), C.byref(st)))
         self.stats = st.as_dict()
-        out, iz, izz = [], 0, 0
-        for n in self.sizes:
-            out.append((z[iz:iz + n].copy(), zz[izz:izz + n - 1].copy()))
-            iz += n
-            izz += n - 1
-        return out
+        oz, ozz = self._z_off, self._zz_off
+        return [(z[oz[i]:oz[i + 1]], zz[ozz[i]:ozz[i + 1]]) for i in range(len(self.sizes))]
 
     def close(self) -> None:
         if self.h:
