@@ -265,18 +265,19 @@ void inline_phase_records(std::vector<VInstr> &ins, std::vector<uint32_t> &ints,
     for (VInstr &v : ins) {
         if ((v.op & VOP_MASK) != V_PHASES) continue;
         // Pairs with register bit r (always the first of the two) come
-        // grouped by r as (b, w) records, inl[1 + r] of them; the other pairs
-        // follow as (a, b, w) records, inl[5] of them, so the kernel walks
-        // each register bit's pairs with that bit fixed at compile time.
+        // grouped by r as (b, w) records, inl[1 + r] of them (r <
+        // VREG_SLOTS); the other pairs follow as (a, b, w) records,
+        // inl[1 + VREG_SLOTS] of them, so the kernel walks each register
+        // bit's pairs with that bit fixed at compile time.
         const uint32_t off = (uint32_t)ints.size();
-        uint32_t counts[5] = {0, 0, 0, 0, 0};
-        for (uint32_t pass = 0; pass < 5; pass++) {
+        uint32_t counts[VREG_SLOTS + 1] = {};
+        for (uint32_t pass = 0; pass <= VREG_SLOTS; pass++) {
             for (uint32_t i = 0; i < v.k; i++) {
                 const uint32_t a = ints[v.aux + 3 * i], b = ints[v.aux + 3 * i + 1], ci = ints[v.aux + 3 * i + 2];
                 require(ci < coefs.size(), "internal: pair phase index out of range");
                 const bool reg = (a & VT_KIND) == VT_REG;
-                require(!reg || (a & 0xffffu) < 4, "internal: register bit out of range in a phase run");
-                if (reg ? (a & 0xffffu) != pass : pass != 4) continue;
+                require(!reg || (a & 0xffffu) < VREG_SLOTS, "internal: register bit out of range in a phase run");
+                if (reg ? (a & 0xffffu) != pass : pass != VREG_SLOTS) continue;
                 uint32_t w[W];
                 std::memcpy(w, &coefs[ci], sizeof(T2));
                 if (!reg) ints.push_back(a);
@@ -286,7 +287,7 @@ void inline_phase_records(std::vector<VInstr> &ins, std::vector<uint32_t> &ints,
             }
         }
         v.inl[0] = off;
-        for (uint32_t r = 0; r < 5; r++) v.inl[1 + r] = counts[r];
+        for (uint32_t r = 0; r <= VREG_SLOTS; r++) v.inl[1 + r] = counts[r];
     }
 }
 

@@ -176,10 +176,11 @@ struct Gen {
         }
         if (sign) {
             v.neg = rr;
-            require(6 + g.w_bits <= 10 && g.r_bits <= 4, "internal: sign masks cover 10 thread and 4 register bits");
+            require(6 + g.w_bits <= 10 && g.r_bits <= VSIGN_TT && VSIGN_TT + 10 <= 16,
+                    "internal: sign masks cover 10 thread and 6 register bits");
             v.form = (has_tr ? 1u : 0u) | (has_tt ? 2u : 0u);
-            for (uint32_t r = 0; r < 4; r++) v.inl[r] = tr_mask[r];
-            for (uint32_t t = 0; t < 10; t++) v.inl[4 + t] = tt_mask[t];
+            for (uint32_t r = 0; r < g.r_bits; r++) v.inl[r] = tr_mask[r];
+            for (uint32_t t = 0; t < 10; t++) v.inl[VSIGN_TT + t] = tt_mask[t];
         } else if (has_rr) {
             v.form = 1;
             v.coef = (uint32_t)coefs.size();
@@ -686,7 +687,7 @@ uint32_t vhandler(const VInstr &v) {
     const bool ctrl = (v.op & VF_CTRL) != 0;
     switch (op) {
         case V_U1R:
-            require(v.slot < 4, "register bit out of range");
+            require(v.slot < VREG_SLOTS, "register bit out of range");
             if (ctrl) return H_U1R_CTRL + v.slot;
             if (v.form == FORM_UNIT_ROT) return H_U1R_ROT + 4 * v.slot + (v.aux & 3);
             if (v.form == FORM_UNIT_DIAG) return H_U1R_DIAG + v.slot;

@@ -46,32 +46,41 @@ constexpr uint32_t VF_CTRL = 1u << 8;   // controls on thread or register bits (
 constexpr uint32_t VF_GCTRL = 1u << 31;  // controls on non-tile qubits (gmask nonzero): the sign bit, one compare
 constexpr uint32_t VH_SHIFT = 16;
 
+// Register bits a handler can name (the geometries use 4 or 5).
+constexpr uint32_t VREG_SLOTS = 5;
+
 // Kernel code paths. Each fixes at compile time what the VOp, form, slot and
 // rotation leave open, so an instruction costs one switch in the kernel.
 enum VHandler : uint32_t {
     H_U1R_ROT = 0,    // + 4 * register bit + k          (FORM_UNIT_ROT)
-    H_U1R_DIAG = 16,  // + register bit
-    H_U1R_ANTI = 20,  // + register bit
-    H_U1R_FULL = 24,  // + register bit
-    H_U1R_CTRL = 28,  // + register bit                  (full form, controls)
-    H_U1L_ROT = 32,   // + 4 * lane bit + k              (FORM_UNIT_ROT)
-    H_U1L_DIAG = 56,  // + lane bit
-    H_U1L_ANTI = 62,  // + lane bit
-    H_U1L_FULL = 68,  // + lane bit
-    H_U1L_CTRL = 74,  // + lane bit                      (full form, controls)
-    H_DIAG = 80,      // + 2 * threaded + (form == 1)
-    H_SIGNS = 84,
-    H_PHASES = 85,
-    H_SCALE1 = 86,    // SCALE form 1: one uniform factor
-    H_SCALE = 87,
-    H_XCHG = 88,
-    H_LDSOP = 89,
-    H_U1R_IMAG = 90,  // + register bit                  (FORM_UNIT_IMAG)
-    H_U1R_REAL = 94,  // + register bit                  (FORM_UNIT_REAL)
-    H_U1L_IMAG = 98,  // + lane bit
-    H_U1L_REAL = 104, // + lane bit
-    H_COUNT = 110,
+    H_U1R_DIAG = 20,  // + register bit
+    H_U1R_ANTI = 25,  // + register bit
+    H_U1R_FULL = 30,  // + register bit
+    H_U1R_CTRL = 35,  // + register bit                  (full form, controls)
+    H_U1L_ROT = 40,   // + 4 * lane bit + k              (FORM_UNIT_ROT)
+    H_U1L_DIAG = 64,  // + lane bit
+    H_U1L_ANTI = 70,  // + lane bit
+    H_U1L_FULL = 76,  // + lane bit
+    H_U1L_CTRL = 82,  // + lane bit                      (full form, controls)
+    H_DIAG = 88,      // + 2 * threaded + (form == 1)
+    H_SIGNS = 92,
+    H_PHASES = 93,
+    H_SCALE1 = 94,    // SCALE form 1: one uniform factor
+    H_SCALE = 95,
+    H_XCHG = 96,
+    H_LDSOP = 97,
+    H_U1R_IMAG = 98,  // + register bit                  (FORM_UNIT_IMAG)
+    H_U1R_REAL = 103, // + register bit                  (FORM_UNIT_REAL)
+    H_U1L_IMAG = 108, // + lane bit
+    H_U1L_REAL = 114, // + lane bit
+    H_COUNT = 120,
 };
+static_assert(H_U1R_DIAG == 4 * VREG_SLOTS && H_U1R_IMAG + 2 * VREG_SLOTS == H_U1L_IMAG, "register-bit handler blocks");
+
+// A V_SIGNS instruction's masks in VInstr::inl: inl[r] (r < VSIGN_TT) the
+// thread bits that flip register bit r's linear term, inl[VSIGN_TT + t]
+// (t < 10) the thread bits u > t whose product with bit t enters the parity.
+constexpr uint32_t VSIGN_TT = 6;
 
 // Forms of a 1-qubit gate's coefficients c[0..3]:
 //   FORM_FULL      [[c0, c1], [c2, c3]]
@@ -131,7 +140,7 @@ struct alignas(64) VInstr {
 };
 
 // The handler of an instruction (VF_* flags and fields already set); the
-// register-bit count must be at most 4.
+// register-bit count must be at most VREG_SLOTS.
 uint32_t vhandler(const VInstr &v);
 
 static_assert(offsetof(VInstr, neg) == 24 && offsetof(VInstr, inl) == 32 && sizeof(VInstr) % 64 == 0,

@@ -237,7 +237,7 @@ __device__ __forceinline__ void signs(T2 (&x)[1 << RB], const QVD_CONST VInstr &
     if (form & 2) {
 #pragma unroll
         for (int t = 0; t < 10; t++) {
-            const uint32_t m = v.inl[4 + t];
+            const uint32_t m = v.inl[VSIGN_TT + t];
             if (m != 0) al ^= (tid >> t) & __builtin_popcount(tid & m);
         }
         al &= 1u;
@@ -474,10 +474,10 @@ __device__ __forceinline__ void phases(T2 (&x)[1 << RB], const QVD_CONST VInstr 
     for (int b = 0; b < RB; b++) pr[b] = one;
     // Pair records in the state's precision, built at upload: the pairs of
     // register bit r as (b, w), inl[1 + r] of them, then the others as
-    // (a, b, w), inl[5] of them. One scalar fetch per pair, none depending
-    // on another load, and no dispatch on the register bit.
+    // (a, b, w), inl[1 + VREG_SLOTS] of them. One scalar fetch per pair, none
+    // depending on another load, and no dispatch on the register bit.
     constexpr uint32_t W = sizeof(T2) / 4;
-    static_assert(RB == 4, "the host writes one pair group per register bit 0-3");
+    static_assert(RB <= (int)VREG_SLOTS, "the host writes one pair group per register bit below VREG_SLOTS");
     auto weight = [&](const QVD_CONST uint32_t *r) {
         T2 w;
         if constexpr (sizeof(T2) == 8) {
@@ -499,7 +499,7 @@ __device__ __forceinline__ void phases(T2 (&x)[1 << RB], const QVD_CONST VInstr 
         const uint32_t n = v.inl[1 + rb];
         for (uint32_t i = 0; i < n; i++, rec += 1 + W) pr[rb] = cm(sel(bit(rec[0]), weight(rec + 1), one), pr[rb]);
     }
-    const uint32_t n_other = v.inl[5];
+    const uint32_t n_other = v.inl[1 + VREG_SLOTS];
     for (uint32_t i = 0; i < n_other; i++, rec += 2 + W) t = cm(sel(bit(rec[0]) && bit(rec[1]), weight(rec + 2), one), t);
     // f[j] is live only until its last child; the barriers keep the
     // scheduler from hoisting every product ahead of its use.
@@ -776,8 +776,8 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
                 inline_coefs(v, c);
                 if constexpr (H < H_U1L_IMAG) {
                     constexpr int F = H < H_U1R_REAL ? FORM_UNIT_IMAG : FORM_UNIT_REAL;
-                    constexpr int B = (H - H_U1R_IMAG) % 4;
-                    u1r<T2, RB, B, F, false>(x, c, 1, 0, 0);
+                    constexpr int B = (H - H_U1R_IMAG) % VREG_SLOTS;
+                    if constexpr (B < RB) u1r<T2, RB, B, F, false>(x, c, 1, 0, 0);
                 } else {
                     constexpr int F = H < H_U1L_REAL ? FORM_UNIT_IMAG : FORM_UNIT_REAL;
                     constexpr int B = (H - H_U1L_IMAG) % 6;
@@ -786,18 +786,18 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
                 }
             } else if constexpr (H < H_U1R_DIAG) {
                 constexpr int B = (H - H_U1R_ROT) / 4, K = (H - H_U1R_ROT) % 4;
-                u1r<T2, RB, B, FORM_UNIT_ROT, false, K>(x, nullptr, 1, 0, 0);
+                if constexpr (B < RB) u1r<T2, RB, B, FORM_UNIT_ROT, false, K>(x, nullptr, 1, 0, 0);
             } else if constexpr (H < H_U1R_CTRL) {
                 constexpr int F = H < H_U1R_ANTI ? FORM_UNIT_DIAG : H < H_U1R_FULL ? FORM_UNIT_ANTI : FORM_FULL;
-                constexpr int B = (H - H_U1R_DIAG) % 4;
+                constexpr int B = (H - H_U1R_DIAG) % VREG_SLOTS;
                 T2 c[4];
                 inline_coefs(v, c);
-                u1r<T2, RB, B, F, false>(x, c, 1, 0, 0);
+                if constexpr (B < RB) u1r<T2, RB, B, F, false>(x, c, 1, 0, 0);
             } else if constexpr (H < H_U1L_ROT) {
                 constexpr int B = H - H_U1R_CTRL;
                 T2 c[4];
                 inline_coefs(v, c);
-                u1r<T2, RB, B, FORM_FULL, true>(x, c, (tid & v.tmask) == v.tval, v.rmask, v.rval);
+                if constexpr (B < RB) u1r<T2, RB, B, FORM_FULL, true>(x, c, (tid & v.tmask) == v.tval, v.rmask, v.rval);
             } else if constexpr (H < H_U1L_DIAG) {
                 constexpr int B = (H - H_U1L_ROT) / 4, K = (H - H_U1L_ROT) % 4;
                 if constexpr (B < 5) u1l_bit_rot<T2, RB, B, K>(x, lane);
@@ -880,7 +880,7 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
                 }
             }
         };
-        static_assert(H_COUNT == 110, "the switch below lists every handler");
+        static_assert(H_COUNT == 120, "the switch below lists every handler");
 #define QVD_H(n) \
     case (n): handle(std::integral_constant<uint32_t, (n)>{}, v); break;
 #define QVD_H4(n) QVD_H(n) QVD_H((n) + 1) QVD_H((n) + 2) QVD_H((n) + 3)
@@ -890,8 +890,8 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
             switch ((w >> VH_SHIFT) & 0xffu) {
                 QVD_H4(0) QVD_H4(4) QVD_H4(8) QVD_H4(12) QVD_H4(16) QVD_H4(20) QVD_H4(24) QVD_H4(28)
                 QVD_H4(32) QVD_H4(36) QVD_H4(40) QVD_H4(44) QVD_H4(48) QVD_H4(52) QVD_H4(56) QVD_H4(60)
-                QVD_H4(64) QVD_H4(68) QVD_H4(72) QVD_H4(76) QVD_H4(80) QVD_H4(84) QVD_H(88) QVD_H(89)
-                QVD_H4(90) QVD_H4(94) QVD_H4(98) QVD_H4(102) QVD_H4(106)
+                QVD_H4(64) QVD_H4(68) QVD_H4(72) QVD_H4(76) QVD_H4(80) QVD_H4(84) QVD_H4(88) QVD_H4(92)
+                QVD_H4(96) QVD_H4(100) QVD_H4(104) QVD_H4(108) QVD_H4(112) QVD_H4(116)
                 default: break;
             }
         }
@@ -919,6 +919,18 @@ __device__ __forceinline__ void stage_tiles(T2 *__restrict__ s, const QVD_CONST 
         }
     }
 }
+
+// The complex64 geometry: 2^C64_WB waves of 2^C64_RB registers per thread
+// (a tile of 2^(6 + C64_WB + C64_RB) = 2^14 amplitudes). 16 waves of 16
+// registers (4 waves per SIMD, 110 VGPRs) beat 8 waves of 32 registers
+// (2 waves per SIMD, 244 VGPRs) on every workload measured, although the
+// latter decodes each instruction once for twice the amplitudes: 61.3 ms
+// against 66.7 ms on the 30-qubit random circuit, 5.3 against 6.5 ms on
+// QFT-30. QVD_C64_RB=5 builds the latter.
+#ifndef QVD_C64_RB
+#define QVD_C64_RB 4
+#endif
+constexpr int C64_RB = QVD_C64_RB, C64_WB = 8 - C64_RB;
 
 }  // namespace
 
@@ -1031,7 +1043,7 @@ __global__ __launch_bounds__(64 << WB) void k_vstage_queue(T2 *__restrict__ s, c
 
 template <typename T2>
 VGeom vstage_geom() {
-    return sizeof(T2) == 8 ? VGeom{4, 4, 4} : VGeom{3, 4, 3};
+    return sizeof(T2) == 8 ? VGeom{C64_WB, C64_RB, 4} : VGeom{3, 4, 3};
 }
 
 template <typename T2>
@@ -1053,10 +1065,10 @@ hipError_t launch_vstage(T2 *s, const VStageDesc *descs, uint32_t ndesc, const u
     // descriptors.
     Kernel k, kz, kf, kfz;
     if constexpr (sizeof(T2) == 8) {
-        k = k_vstage<T2, 4, 4, 4, false>;
-        kz = k_vstage<T2, 4, 4, 4, true>;
-        kf = k_vstage<T2, 4, 4, -1, false>;
-        kfz = k_vstage<T2, 4, 4, -1, true>;
+        k = k_vstage<T2, C64_WB, C64_RB, 4, false>;
+        kz = k_vstage<T2, C64_WB, C64_RB, 4, true>;
+        kf = k_vstage<T2, C64_WB, C64_RB, -1, false>;
+        kfz = k_vstage<T2, C64_WB, C64_RB, -1, true>;
     } else {
         k = k_vstage<T2, 3, 4, 3, false>;
         kz = k_vstage<T2, 3, 4, 3, true>;
@@ -1094,7 +1106,7 @@ namespace {
 
 template <typename T2>
 auto vstage_queue_kernel() {
-    if constexpr (sizeof(T2) == 8) return k_vstage_queue<T2, 4, 4, 4>;
+    if constexpr (sizeof(T2) == 8) return k_vstage_queue<T2, C64_WB, C64_RB, 4>;
     else return k_vstage_queue<T2, 3, 4, 3>;
 }
 
