@@ -551,8 +551,9 @@ __global__ __launch_bounds__(1 << EZ_TID_BITS) void k_expect_segments(const T2 *
 }
 
 // Combines the segment sums into z[q] = sum p (1 - 2 bit q) and
-// zz[q] = sum p (1 - 2 (bit q xor bit q+1)). One thread per output.
-__global__ void k_expect_combine(const double *__restrict__ seg_all, ExpDesc one, const ExpDesc *__restrict__ descs,
+// zz[q] = sum p (1 - 2 (bit q xor bit q+1)). One thread per output, in
+// blocks of 64 threads.
+__global__ __launch_bounds__(64) void k_expect_combine(const double *__restrict__ seg_all, ExpDesc one, const ExpDesc *__restrict__ descs,
                                  uint32_t ndesc, const uint32_t *__restrict__ out_start, double *__restrict__ z,
                                  double *__restrict__ zz) {
     const uint32_t wg = threadIdx.x + blockIdx.x * blockDim.x;
@@ -567,21 +568,53 @@ __global__ void k_expect_combine(const double *__restrict__ seg_all, ExpDesc one
     const uint32_t n = d.n, B = d.B;
     if (w >= 2 * n - 1) return;
     const double *seg_sums = seg_all + d.seg_off;
+    // What the output takes from segment g besides its total T = o[0]:
+    //   0: o[idx]                          (its bit, or both bits of its pair, inside the segment)
+    //   1: T where bit sh of g is set      (<Z_q>, q at or above the segment bits)
+    //   2: (g & 1) ? T - o[idx] : o[idx]   (<Z_q Z_q+1>, bit q the segment's top bit)
+    //   3: T where bits sh and sh + 1 of g differ
+    uint32_t mode, idx = 0, sh = 0;
+    if (w < n) {
+        const uint32_t q = w;
+        if (q < B) mode = 0, idx = 1 + q;
+        else mode = 1, sh = q - B;
+    } else {
+        const uint32_t q = w - n;
+        if (q + 1 < B) mode = 0, idx = 1 + B + q;
+        else if (q + 1 == B) mode = 2, idx = 1 + q;
+        else mode = 3, sh = q - B;
+    }
+    // Branch-free: where an output takes nothing from a segment it adds
+    // +0.0, which leaves the sum as it is (both sums start at +0.0 and add
+    // non-negative values, so neither is ever -0.0).
     double tot = 0, part = 0;
-    for (uint64_t g = 0; g < d.nsegs; g++) {
-        const double *o = seg_sums + g * (2 * B);
-        const double T = o[0];
+    auto add = [&](uint64_t g, double T, double V) {
         tot += T;
-        if (w < n) {
-            const uint32_t q = w;
-            if (q < B) part += o[1 + q];
-            else if ((g >> (q - B)) & 1) part += T;
-        } else {
-            const uint32_t q = w - n;
-            if (q + 1 < B) part += o[1 + B + q];
-            else if (q + 1 == B) part += ((g & 1) ? T - o[1 + q] : o[1 + q]);
-            else if (((g >> (q - B)) ^ (g >> (q + 1 - B))) & 1) part += T;
+        const bool b0 = (g >> sh) & 1, b1 = (g >> (sh + 1)) & 1;
+        const double a = mode == 0 ? V : mode == 1 ? (b0 ? T : 0.0) : mode == 2 ? ((g & 1) ? T - V : V) : (b0 != b1 ? T : 0.0);
+        part += a;
+    };
+    // The segments in order; G segments' values are loaded at a time, so a
+    // state of many segments does not wait for one load per segment.
+    // One pointer steps through a batch (precomputed per-segment addresses
+    // would be loop invariants the compiler keeps, and spills).
+    constexpr uint32_t G = 32;
+    const uint64_t stride = 2 * (uint64_t)B;
+    uint64_t g = 0;
+    for (; g + G <= d.nsegs; g += G) {
+        double T[G], V[G];
+        const double *o = seg_sums + g * stride;
+#pragma unroll
+        for (uint32_t u = 0; u < G; u++, o += stride) {
+            T[u] = o[0];
+            V[u] = o[idx];
         }
+#pragma unroll
+        for (uint32_t u = 0; u < G; u++) add(g + u, T[u], V[u]);
+    }
+    for (; g < d.nsegs; g++) {
+        const double *o = seg_sums + g * stride;
+        add(g, o[0], o[idx]);
     }
     if (w < n) z[d.z_off + w] = tot - 2 * part;
     else zz[d.zz_off + w - n] = tot - 2 * part;
