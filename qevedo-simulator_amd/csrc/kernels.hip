@@ -326,7 +326,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_masked_norm(const T2 *__restric
     __shared__ double scratch[RED_THREADS / 64];
     double acc = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride)
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // U loads in flight per thread, summed in index order. An amplitude
+    // outside the mask adds +0.0, which leaves the sum (never -0.0) as it is;
+    // its load re-reads the batch's first address (cached) instead, so the
+    // loads need no branch, which would wait for each of them.
+    constexpr int U = 8;
+    for (; i + (U - 1) * stride < len; i += U * stride) {
+        T2 a[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t k = i + u * stride;
+            in[u] = (k & mask) == val;
+            a[u] = s[in[u] ? k : i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += in[u] ? prob(a[u]) : 0.0;
+    }
+    for (; i < len; i += stride)
         if ((i & mask) == val) acc += prob(s[i]);
     const double r = block_sum(acc, scratch);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
