@@ -361,34 +361,49 @@ __global__ void k_probabilities(const T2 *__restrict__ s, uint64_t len, double *
 // their Z parts z[t] (qubits with Z or Y): per block, the sums over j of
 // (-1)^popcount((j ^ x) & z[t]) conj(a_j) a_(j ^ x), real and imaginary
 // parts in FP64, written to partial[block * 2 * nterms + 2 t (+1)]. With the
-// phase i^popcount(x & z[t]) applied afterwards this is <P_t>.
-template <typename T2>
+// phase i^popcount(x & z[t]) applied afterwards this is <P_t>. KT >= nterms
+// slots are computed (a power of two, so a group of few terms does not pay
+// for PAULI_TERMS).
+template <typename T2, int KT>
 __global__ __launch_bounds__(RED_THREADS) void k_expect_pauli(const T2 *__restrict__ s, uint64_t len, uint64_t x,
                                                               const uint64_t *__restrict__ zs, uint32_t nterms,
                                                               double *__restrict__ partial) {
     __shared__ double scratch[RED_THREADS / 64];
-    double re[PAULI_TERMS], im[PAULI_TERMS];
-    uint64_t z[PAULI_TERMS];
+    double re[KT], im[KT];
+    uint64_t z[KT];
 #pragma unroll
-    for (int t = 0; t < PAULI_TERMS; t++) {
+    for (int t = 0; t < KT; t++) {
         re[t] = im[t] = 0;
         z[t] = t < (int)nterms ? zs[t] : 0;
     }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += stride) {
-        const T2 a = s[j], b = s[j ^ x];
+    auto add = [&](uint64_t j, const T2 a, const T2 b) {
         const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
         const double vr = ar * br + ai * bi, vi = ar * bi - ai * br;  // conj(a) b
         const uint64_t k = j ^ x;
 #pragma unroll
-        for (int t = 0; t < PAULI_TERMS; t++) {  // every slot: unused ones have z = 0 and are not written
+        for (int t = 0; t < KT; t++) {  // every slot: unused ones have z = 0 and are not written
             const bool neg = __popcll(k & z[t]) & 1;
             re[t] += neg ? -vr : vr;
             im[t] += neg ? -vi : vi;
         }
-    }
+    };
+    // U iterations' loads in flight per thread, summed in index order.
+    constexpr int U = 4;
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; j + (U - 1) * stride < len; j += U * stride) {
+        T2 a[U], b[U];
 #pragma unroll
-    for (int t = 0; t < PAULI_TERMS; t++) {
+        for (int u = 0; u < U; u++) {
+            a[u] = s[j + u * stride];
+            b[u] = s[(j + u * stride) ^ x];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) add(j + u * stride, a[u], b[u]);
+    }
+    for (; j < len; j += stride) add(j, s[j], s[j ^ x]);
+#pragma unroll
+    for (int t = 0; t < KT; t++) {
         const double r = block_sum(re[t], scratch);
         const double i = block_sum(im[t], scratch);
         if (threadIdx.x == 0 && t < (int)nterms) {
@@ -1108,7 +1123,11 @@ void launch_probabilities(const T2 *s, uint64_t len, double *out, hipStream_t st
 template <typename T2>
 void launch_expect_pauli(const T2 *s, uint64_t len, uint64_t x, const uint64_t *zs, uint32_t nterms, double *partial,
                          unsigned blocks, double *out, hipStream_t st) {
-    k_expect_pauli<T2><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
+    if (nterms <= 1) k_expect_pauli<T2, 1><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
+    else if (nterms <= 2) k_expect_pauli<T2, 2><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
+    else if (nterms <= 4) k_expect_pauli<T2, 4><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
+    else if (nterms <= 8) k_expect_pauli<T2, 8><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
+    else k_expect_pauli<T2, PAULI_TERMS><<<blocks, RED_THREADS, 0, st>>>(s, len, x, zs, nterms, partial);
     k_sum_columns<<<1, 64, 0, st>>>(partial, blocks, 2 * nterms, out);
 }
 
