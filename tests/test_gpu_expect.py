@@ -92,3 +92,39 @@ def test_probabilities_on_the_device(precision):
             got = s.probabilities()
         assert np.max(np.abs(got - p)) < 1e-15
         assert abs(got.sum() - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("precision,n", [("single", 23), ("double", 22)])
+def test_large_state_reductions(precision, n):
+    """States large enough for the reductions' batched loads (norm and
+    probabilities over 8 strided loads per thread, the <Z> combine over 32
+    segments at a time, Pauli strings over 4 iterations): norm_sqr,
+    probability_one, <Z>, <ZZ> and Pauli strings against NumPy in FP64 on the
+    stored amplitudes."""
+    r = rng(33)
+    psi = random_state(r, n)
+    with StateVector(n, precision) as s:
+        s.set_amplitudes(psi)
+        a = s.amplitudes()
+        p = np.abs(a) ** 2
+        idx = np.arange(1 << n, dtype=np.uint64)
+        print(f"large reductions n={n} {precision}", flush=True)
+        assert abs(s.norm_sqr() - p.sum()) < 1e-12
+        for q in (0, 5, 17, n - 1):
+            bit = (idx >> np.uint64(q)) & np.uint64(1)
+            assert abs(s.probability_one(q) - p[bit == 1].sum()) < 1e-12, q
+        z, zz = s.expect_z()
+        for q in range(n):
+            bit = ((idx >> np.uint64(q)) & np.uint64(1)).astype(np.int64)
+            assert abs(z[q] - np.sum(p * (1 - 2 * bit))) < 1e-12, q
+            if q + 1 < n:
+                par = bit ^ ((idx >> np.uint64(q + 1)) & np.uint64(1)).astype(np.int64)
+                assert abs(zz[q] - np.sum(p * (1 - 2 * par))) < 1e-12, q
+        xs = [0, 0b101, 1 << (n - 1), (1 << (n - 1)) | 3]
+        zs = [0b11, 0b110, 1 << (n - 2), (1 << (n - 1)) | 1]
+        got = s.expect_pauli(xs, zs)
+    for t, (x, zm) in enumerate(zip(xs, zs)):
+        k = idx ^ np.uint64(x)
+        sign = 1 - 2 * (np.bitwise_count(k & np.uint64(zm)) & 1).astype(np.int64)
+        v = np.sum(np.conj(a) * a[k.astype(np.int64)] * sign) * (1j ** int(bin(x & zm).count("1")))
+        assert abs(got[t] - v.real) < 1e-12, t
